@@ -1,0 +1,150 @@
+/*
+ * llh.h - C ABI of linfa-linalg-hip: the Cholesky / SPD-solve / triangular-solve
+ * path of linfa-linalg 0.2.1, computed by hand-written HIP kernels on one
+ * MI355X (gfx950).
+ *
+ * The reference's interface is a set of Rust trait extension methods on
+ * ndarray::ArrayBase<S, Ix2> with A: NdFloat (f32 | f64).  Each "plain" entry
+ * point below replaces one of those methods for host-memory arrays of any
+ * (signed) element strides, so a Rust binding can pass `arr.as_mut_ptr()`,
+ * `arr.dim()` and `arr.strides()` straight through:
+ *
+ *   element (i, j) of a view lives at  ptr[i * rs + j * cs]
+ *
+ * and `ptr` addresses element (0, 0) (what ndarray's as_ptr returns, also for
+ * inverted axes).  The in-place and "dirty" semantics of the reference are kept:
+ * only the lower triangle of A is read, the dirty forms leave the strict upper
+ * triangle untouched, and solvec/invc leave A holding the dirty factor.
+ *
+ * Errors mirror LinalgError (lib.rs:33-60): every entry point returns an
+ * llh_status and, when `err` is non-null, fills it with the payload.  Checks run
+ * in the reference's order: NotSquare, then NotPositiveDefinite, then WrongRows.
+ *
+ * The "dev" entry points take device-resident row-major matrices (element (i, j)
+ * at ptr[i * ld + j]) and are what the throughput figures time.  The factor,
+ * batch and server objects are extensions with no counterpart in the reference.
+ *
+ * Threading: a context owns one HIP stream; calls on one context must not
+ * overlap (lock externally to share it across threads).  Calls return after the
+ * work is complete unless stated otherwise.
+ */
+#ifndef LLH_H
+#define LLH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LLH_ABI_VERSION 1
+
+typedef enum llh_status {
+  LLH_OK = 0,
+  LLH_ERR_NOT_SQUARE = 1,             /* LinalgError::NotSquare{rows, cols}     lib.rs:37-38 */
+  LLH_ERR_NOT_POSITIVE_DEFINITE = 2,  /* LinalgError::NotPositiveDefinite        lib.rs:43-44 */
+  LLH_ERR_WRONG_ROWS = 3,             /* LinalgError::WrongRows{expected,actual} lib.rs:55-56 */
+  /* Extensions (LinalgError is #[non_exhaustive], lib.rs:34): */
+  LLH_ERR_INVALID_ARGUMENT = 16,      /* null pointer, bad leading dimension, bad option */
+  LLH_ERR_HIP = 17,                   /* a HIP runtime call failed; see hip_error */
+  LLH_ERR_OUT_OF_MEMORY = 18,         /* device allocation failed / over the HBM budget */
+  LLH_ERR_NO_CONVERGENCE = 19,        /* iterative refinement stalled (result still returned) */
+  LLH_ERR_REJECTED = 20               /* server admission refused the request */
+} llh_status;
+
+typedef enum llh_uplo { LLH_UPPER = 0, LLH_LOWER = 1 } llh_uplo; /* UPLO, triangular.rs:10-13 */
+
+typedef struct llh_error {
+  int32_t code;      /* llh_status */
+  int32_t hip_error; /* hipError_t when code == LLH_ERR_HIP */
+  uint64_t a;        /* NotSquare: rows;  WrongRows: expected */
+  uint64_t b;        /* NotSquare: cols;  WrongRows: actual   */
+  int64_t column;    /* NotPositiveDefinite: first row j whose pivot fails `d <= 0`; else -1 */
+  char message[192];
+} llh_error;
+
+typedef struct llh_context llh_context;
+
+/* One context per (thread, device).  device < 0 uses the current device. */
+llh_status llh_context_create(int device, llh_context** out, llh_error* err);
+void llh_context_destroy(llh_context* ctx);
+/* Run on a caller-owned hipStream_t instead of the context's own stream. */
+llh_status llh_context_set_stream(llh_context* ctx, void* hip_stream, llh_error* err);
+void* llh_context_stream(llh_context* ctx);
+const char* llh_status_string(int32_t code);
+
+/* ---------------------------------------------------------------------------
+ * Plain forms: host memory, any strides.  _d = f64, _s = f32.
+ * ------------------------------------------------------------------------- */
+#define LLH_PLAIN_DECL(T, sfx)                                                            \
+  /* CholeskyInplace::cholesky_inplace_dirty, cholesky.rs:19 (impl :51-76).  On          \
+     NotPositiveDefinite at row j the array is left as the reference leaves it: rows    \
+     < j hold L, row j holds L[j, ..j] and its original diagonal, later rows are        \
+     untouched. */                                                                       \
+  llh_status llh_cholesky_inplace_dirty_##sfx(llh_context* ctx, size_t rows, size_t cols, \
+                                              T* a, ptrdiff_t rs, ptrdiff_t cs,           \
+                                              llh_error* err);                            \
+  /* CholeskyInplace::cholesky_inplace, cholesky.rs:33 (impl :78-82): dirty, then         \
+     triangular_inplace(Lower). */                                                       \
+  llh_status llh_cholesky_inplace_##sfx(llh_context* ctx, size_t rows, size_t cols, T* a,  \
+                                        ptrdiff_t rs, ptrdiff_t cs, llh_error* err);      \
+  /* IntoTriangular::triangular_inplace, triangular.rs:19 (impl :37-53). */              \
+  llh_status llh_triangular_inplace_##sfx(llh_context* ctx, size_t rows, size_t cols,     \
+                                          T* a, ptrdiff_t rs, ptrdiff_t cs,               \
+                                          llh_uplo uplo, llh_error* err);                 \
+  /* Triangular::is_triangular, triangular.rs:59 (impl :67-90).  *out = 1 or 0. */       \
+  llh_status llh_is_triangular_##sfx(llh_context* ctx, size_t rows, size_t cols,          \
+                                     const T* a, ptrdiff_t rs, ptrdiff_t cs,              \
+                                     llh_uplo uplo, int* out, llh_error* err);            \
+  /* SolveTriangularInplace::solve_triangular_inplace, triangular.rs:149 (impl           \
+     :161-168, kernel :95-128).  Reads only the `uplo` triangle and the diagonal of a;   \
+     a zero pivot yields inf/NaN, not an error (triangular.rs:274-276). */               \
+  llh_status llh_solve_triangular_inplace_##sfx(                                           \
+      llh_context* ctx, size_t rows, size_t cols, const T* a, ptrdiff_t rs, ptrdiff_t cs, \
+      size_t b_rows, size_t b_cols, T* b, ptrdiff_t brs, ptrdiff_t bcs, llh_uplo uplo,   \
+      llh_error* err);                                                                    \
+  /* SolveCInplace::solvec_inplace, cholesky.rs:122 (impl :136-144).  A is left         \
+     holding the dirty factor, also when the later WrongRows check fails. */             \
+  llh_status llh_solvec_inplace_##sfx(llh_context* ctx, size_t rows, size_t cols, T* a,    \
+                                      ptrdiff_t rs, ptrdiff_t cs, size_t b_rows,          \
+                                      size_t b_cols, T* b, ptrdiff_t brs, ptrdiff_t bcs,  \
+                                      llh_error* err);                                    \
+  /* InverseCInplace::invc_inplace, cholesky.rs:172 (impl :178-182): out (rows x rows,   \
+     strides ors/ocs) receives A^-1; A is left holding the dirty factor. */              \
+  llh_status llh_invc_inplace_##sfx(llh_context* ctx, size_t rows, size_t cols, T* a,      \
+                                    ptrdiff_t rs, ptrdiff_t cs, T* out, ptrdiff_t ors,    \
+                                    ptrdiff_t ocs, llh_error* err);
+
+LLH_PLAIN_DECL(double, d)
+LLH_PLAIN_DECL(float, s)
+#undef LLH_PLAIN_DECL
+
+/* ---------------------------------------------------------------------------
+ * Device-resident forms (row-major, ld >= n; pointers are device memory).
+ * ------------------------------------------------------------------------- */
+#define LLH_DEV_DECL(T, sfx)                                                              \
+  /* Blocked right-looking POTRF of the lower triangle, in place (the device-memory     \
+     counterpart of cholesky_inplace_dirty / cholesky_inplace, cholesky.rs:51-82).      \
+     zero_upper != 0 also zeroes the strict upper triangle. */                          \
+  llh_status llh_dev_potrf_##sfx(llh_context* ctx, size_t n, T* a, size_t lda,            \
+                                 int zero_upper, llh_error* err);                          \
+  /* Solves op(A) X = B in place for m right-hand sides (triangular.rs:95-144).         \
+     uplo names the triangle of op(A); trans != 0 means op(A) = A^T, so the solve       \
+     reads the opposite triangle of A's storage (how solvec solves with L.t(),          \
+     cholesky.rs:142).  B is n x m row-major with ldb >= m. */                          \
+  llh_status llh_dev_solve_triangular_##sfx(llh_context* ctx, llh_uplo uplo, int trans,   \
+                                            size_t n, const T* a, size_t lda, size_t m,    \
+                                            T* b, size_t ldb, llh_error* err);             \
+  /* solvec_inplace on device memory (cholesky.rs:136-144): A := dirty factor, B := X. */ \
+  llh_status llh_dev_solvec_##sfx(llh_context* ctx, size_t n, T* a, size_t lda, size_t m,  \
+                                  T* b, size_t ldb, llh_error* err);
+
+LLH_DEV_DECL(double, d)
+LLH_DEV_DECL(float, s)
+#undef LLH_DEV_DECL
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LLH_H */

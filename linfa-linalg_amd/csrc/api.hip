@@ -1,0 +1,505 @@
+// C ABI (include/llh.h): contexts, the plain host-memory forms that mirror
+// linfa-linalg's traits, and the device-resident forms.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "llh.h"
+#include "llh_context.h"
+#include "llh_driver.h"
+#include "llh_kernels.h"
+
+using namespace llh;
+
+namespace llh {
+
+llh_status set_error(llh_error* err, llh_status code, uint64_t a, uint64_t b, int64_t column,
+                     const char* msg, int hip_error) {
+  if (err != nullptr) {
+    err->code = code;
+    err->hip_error = hip_error;
+    err->a = a;
+    err->b = b;
+    err->column = column;
+    std::snprintf(err->message, sizeof(err->message), "%s", msg ? msg : "");
+  }
+  return code;
+}
+
+llh_status ok(llh_error* err) { return set_error(err, LLH_OK, 0, 0, -1, "", 0); }
+
+llh_status hip_fail(llh_error* err, hipError_t e, const char* where) {
+  char buf[160];
+  std::snprintf(buf, sizeof(buf), "%s: %s", where, hipGetErrorString(e));
+  return set_error(err, e == hipErrorOutOfMemory ? LLH_ERR_OUT_OF_MEMORY : LLH_ERR_HIP, 0, 0, -1, buf, (int)e);
+}
+
+llh_status invalid(llh_error* err, const char* msg) {
+  return set_error(err, LLH_ERR_INVALID_ARGUMENT, 0, 0, -1, msg, 0);
+}
+
+llh_status not_square(llh_error* err, size_t rows, size_t cols) {
+  char buf[96];
+  std::snprintf(buf, sizeof(buf), "Matrix of (%zu, %zu) is not square", rows, cols);
+  return set_error(err, LLH_ERR_NOT_SQUARE, rows, cols, -1, buf, 0);
+}
+
+llh_status not_pd(llh_error* err, int64_t col) {
+  return set_error(err, LLH_ERR_NOT_POSITIVE_DEFINITE, 0, 0, col, "Matrix is not positive definite", 0);
+}
+
+llh_status wrong_rows(llh_error* err, size_t expected, size_t actual) {
+  char buf[96];
+  std::snprintf(buf, sizeof(buf), "Matrix must have %zu rows, not %zu", expected, actual);
+  return set_error(err, LLH_ERR_WRONG_ROWS, expected, actual, -1, buf, 0);
+}
+
+hipError_t ctx_buffer(llh_context* ctx, int slot, size_t bytes, void** out) {
+  auto& b = ctx->scratch[slot];
+  if (b.bytes < bytes) {
+    if (b.ptr != nullptr) {
+      hipError_t e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) return e;
+      (void)hipFree(b.ptr);
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    const size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) return e;
+    b.bytes = want;
+  }
+  *out = b.ptr;
+  return hipSuccess;
+}
+
+}  // namespace llh
+
+namespace {
+
+#define CHECK_HIP(expr, where)                           \
+  do {                                                   \
+    hipError_t e_ = (expr);                              \
+    if (e_ != hipSuccess) return hip_fail(err, e_, where); \
+  } while (0)
+
+// Gathers a strided host view into a row-major buffer and back.
+template <typename T>
+void gather(const T* a, ptrdiff_t rs, ptrdiff_t cs, size_t rows, size_t cols, T* out) {
+  for (size_t i = 0; i < rows; ++i) {
+    const T* row = a + (ptrdiff_t)i * rs;
+    T* o = out + i * cols;
+    if (cs == 1) {
+      std::memcpy(o, row, cols * sizeof(T));
+    } else {
+      for (size_t j = 0; j < cols; ++j) o[j] = row[(ptrdiff_t)j * cs];
+    }
+  }
+}
+
+// Scatters row i's columns [0, cend(i)) of a row-major buffer into a view.
+template <typename T, typename F>
+void scatter(const T* in, size_t rows, size_t cols, T* a, ptrdiff_t rs, ptrdiff_t cs, F cend) {
+  for (size_t i = 0; i < rows; ++i) {
+    T* row = a + (ptrdiff_t)i * rs;
+    const T* src = in + i * cols;
+    const size_t ce = cend(i);
+    if (cs == 1) {
+      std::memcpy(row, src, ce * sizeof(T));
+    } else {
+      for (size_t j = 0; j < ce; ++j) row[(ptrdiff_t)j * cs] = src[j];
+    }
+  }
+}
+
+bool bad_ctx(llh_context* ctx) { return ctx == nullptr || ctx->magic != llh_context::kMagic; }
+
+// Factors the packed host copy `h` (n x n row-major) on the device and writes
+// the result back into the user's view with the reference's semantics.
+// Returns the first failing row or -1.
+template <typename T>
+llh_status factor_plain(llh_context* ctx, size_t n, T* a, ptrdiff_t rs, ptrdiff_t cs, bool clean,
+                        std::vector<T>& h, T** d_out, int64_t* fail_out, llh_error* err) {
+  void* dv = nullptr;
+  CHECK_HIP(ctx_buffer(ctx, 0, n * n * sizeof(T), &dv), "device buffer");
+  T* d = static_cast<T*>(dv);
+  CHECK_HIP(hipMemcpyAsync(d, h.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  CHECK_HIP(potrf_enqueue<T>(d, (int64_t)n, (int)n, ctx->d_info, clean, ctx->stream), "potrf");
+  CHECK_HIP(hipMemcpyAsync(h.data(), d, n * n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
+  CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
+            "status");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  const int32_t f = ctx->h_info[0];
+  if (f == kNoFail) {
+    if (clean)
+      scatter(h.data(), n, n, a, rs, cs, [n](size_t) { return n; });
+    else
+      scatter(h.data(), n, n, a, rs, cs, [](size_t i) { return i + 1; });
+    *fail_out = -1;
+  } else {
+    // cholesky.rs:54-72 returns at row f: rows < f are factored, row f holds
+    // L[f, ..f], the rest of the array is as it was.
+    const size_t fr = (size_t)f;
+    scatter(h.data(), fr + 1, n, a, rs, cs, [fr](size_t i) { return i < fr ? i + 1 : fr; });
+    *fail_out = f;
+  }
+  if (d_out) *d_out = d;
+  return LLH_OK;
+}
+
+template <typename T>
+llh_status cholesky_plain(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdiff_t rs, ptrdiff_t cs,
+                          bool clean, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (rows != cols) return not_square(err, rows, cols);
+  const size_t n = rows;
+  if (n == 0) return ok(err);
+  if (a == nullptr) return invalid(err, "null matrix pointer");
+  if (n > 0x7fffffff) return invalid(err, "matrix too large");
+  std::vector<T> h(n * n);
+  gather(a, rs, cs, n, n, h.data());
+  int64_t fail = -1;
+  llh_status st = factor_plain<T>(ctx, n, a, rs, cs, clean, h, nullptr, &fail, err);
+  if (st != LLH_OK) return st;
+  if (fail >= 0) return not_pd(err, fail);
+  return ok(err);
+}
+
+template <typename T>
+llh_status triangular_plain(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdiff_t rs, ptrdiff_t cs,
+                            llh_uplo uplo, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (rows != cols) return not_square(err, rows, cols);
+  const size_t n = rows;
+  if (n == 0) return ok(err);
+  if (a == nullptr) return invalid(err, "null matrix pointer");
+  std::vector<T> h(n * n);
+  gather(a, rs, cs, n, n, h.data());
+  void* dv = nullptr;
+  CHECK_HIP(ctx_buffer(ctx, 0, n * n * sizeof(T), &dv), "device buffer");
+  T* d = static_cast<T*>(dv);
+  CHECK_HIP(hipMemcpyAsync(d, h.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  CHECK_HIP(launch_zero_triangle<T>(d, (int64_t)n, (int)n, uplo == LLH_LOWER, nullptr, ctx->stream), "zero");
+  CHECK_HIP(hipMemcpyAsync(h.data(), d, n * n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  scatter(h.data(), n, n, a, rs, cs, [n](size_t) { return n; });
+  return ok(err);
+}
+
+template <typename T>
+llh_status is_triangular_plain(llh_context* ctx, size_t rows, size_t cols, const T* a, ptrdiff_t rs,
+                               ptrdiff_t cs, llh_uplo uplo, int* out, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (out == nullptr) return invalid(err, "null output pointer");
+  if (rows != cols) {  // triangular.rs:88-89: non-square is not triangular
+    *out = 0;
+    return ok(err);
+  }
+  const size_t n = rows;
+  if (n <= 1) {
+    *out = 1;
+    return ok(err);
+  }
+  if (a == nullptr) return invalid(err, "null matrix pointer");
+  std::vector<T> h(n * n);
+  gather(a, rs, cs, n, n, h.data());
+  void* dv = nullptr;
+  CHECK_HIP(ctx_buffer(ctx, 0, n * n * sizeof(T), &dv), "device buffer");
+  T* d = static_cast<T*>(dv);
+  CHECK_HIP(hipMemcpyAsync(d, h.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  CHECK_HIP(launch_fill_i32(ctx->d_info + 1, 1, 1, ctx->stream), "flag");
+  CHECK_HIP(launch_is_triangular<T>(d, (int64_t)n, (int)n, uplo == LLH_UPPER, ctx->d_info + 1, ctx->stream),
+            "is_triangular");
+  CHECK_HIP(hipMemcpyAsync(ctx->h_info + 1, ctx->d_info + 1, sizeof(int32_t), hipMemcpyDeviceToHost,
+                           ctx->stream),
+            "flag download");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  *out = ctx->h_info[1] ? 1 : 0;
+  return ok(err);
+}
+
+template <typename T>
+llh_status solve_triangular_plain(llh_context* ctx, size_t rows, size_t cols, const T* a, ptrdiff_t rs,
+                                  ptrdiff_t cs, size_t b_rows, size_t b_cols, T* b, ptrdiff_t brs,
+                                  ptrdiff_t bcs, llh_uplo uplo, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (rows != cols) return not_square(err, rows, cols);  // triangular.rs:102
+  const size_t n = rows;
+  if (b_rows != n) return wrong_rows(err, n, b_rows);  // triangular.rs:103-108
+  if (n == 0 || b_cols == 0) return ok(err);
+  if (a == nullptr || b == nullptr) return invalid(err, "null matrix pointer");
+  std::vector<T> ha(n * n), hb(n * b_cols);
+  gather(a, rs, cs, n, n, ha.data());
+  gather(b, brs, bcs, n, b_cols, hb.data());
+  void *dav = nullptr, *dbv = nullptr;
+  CHECK_HIP(ctx_buffer(ctx, 0, n * n * sizeof(T), &dav), "device buffer");
+  CHECK_HIP(ctx_buffer(ctx, 1, n * b_cols * sizeof(T), &dbv), "device buffer");
+  T* da = static_cast<T*>(dav);
+  T* db = static_cast<T*>(dbv);
+  CHECK_HIP(hipMemcpyAsync(da, ha.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  CHECK_HIP(hipMemcpyAsync(db, hb.data(), n * b_cols * sizeof(T), hipMemcpyHostToDevice, ctx->stream),
+            "upload");
+  CHECK_HIP(trsm_enqueue<T>(uplo == LLH_UPPER, false, (int)n, da, (int64_t)n, (int)b_cols, db,
+                            (int64_t)b_cols, ctx->stream),
+            "trsm");
+  CHECK_HIP(hipMemcpyAsync(hb.data(), db, n * b_cols * sizeof(T), hipMemcpyDeviceToHost, ctx->stream),
+            "download");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  scatter(hb.data(), n, b_cols, b, brs, bcs, [b_cols](size_t) { return b_cols; });
+  return ok(err);
+}
+
+// Small systems go through the exact-order fused kernel (bit-identical to the
+// reference's factor and both sweeps); larger ones through the blocked path.
+template <typename T>
+llh_status solve_on_device(llh_context* ctx, size_t n, T* da, size_t m, T* db, llh_error* err) {
+  if ((int)n <= small_max<T>()) {
+    SmallDesc<T> desc{da, db, (int64_t)n, (int64_t)m, (int32_t)n, (int32_t)m, ctx->d_info};
+    void* ddv = nullptr;
+    CHECK_HIP(ctx_buffer(ctx, 2, sizeof(desc), &ddv), "descriptor buffer");
+    CHECK_HIP(hipMemcpyAsync(ddv, &desc, sizeof(desc), hipMemcpyHostToDevice, ctx->stream), "descriptor");
+    CHECK_HIP(launch_potrf_potrs_exact_batched<T>(static_cast<SmallDesc<T>*>(ddv), 1, (int)n, 1, ctx->stream),
+              "small factor+solve");
+  } else {
+    CHECK_HIP(potrf_enqueue<T>(da, (int64_t)n, (int)n, ctx->d_info, false, ctx->stream), "potrf");
+    CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, ctx->stream),
+              "forward solve");
+    CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, ctx->stream),
+              "back solve");
+  }
+  return LLH_OK;
+}
+
+template <typename T>
+llh_status solvec_plain(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdiff_t rs, ptrdiff_t cs,
+                        size_t b_rows, size_t b_cols, T* b, ptrdiff_t brs, ptrdiff_t bcs, T* inv_out,
+                        ptrdiff_t ors, ptrdiff_t ocs, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (rows != cols) return not_square(err, rows, cols);  // cholesky.rs:52 via :140
+  const size_t n = rows;
+  const bool invert = inv_out != nullptr;
+  const size_t m = invert ? n : b_cols;
+  if (n == 0) {
+    if (!invert && b_rows != 0) return wrong_rows(err, 0, b_rows);
+    return ok(err);
+  }
+  if (a == nullptr || (!invert && b == nullptr && b_cols > 0)) return invalid(err, "null matrix pointer");
+  if (n > 0x7fffffff || m > 0x7fffffff) return invalid(err, "matrix too large");
+  std::vector<T> ha(n * n);
+  gather(a, rs, cs, n, n, ha.data());
+  const bool rows_ok = invert || b_rows == n;
+  if (!rows_ok || m == 0) {
+    // The factor runs (and A keeps the dirty factor) before WrongRows is
+    // reported, cholesky.rs:140-141 with triangular.rs:103-108.
+    int64_t fail = -1;
+    llh_status st = factor_plain<T>(ctx, n, a, rs, cs, false, ha, nullptr, &fail, err);
+    if (st != LLH_OK) return st;
+    if (fail >= 0) return not_pd(err, fail);
+    if (!rows_ok) return wrong_rows(err, n, b_rows);
+    return ok(err);
+  }
+  std::vector<T> hb(n * m);
+  void *dav = nullptr, *dbv = nullptr;
+  CHECK_HIP(ctx_buffer(ctx, 0, n * n * sizeof(T), &dav), "device buffer");
+  CHECK_HIP(ctx_buffer(ctx, 1, n * m * sizeof(T), &dbv), "device buffer");
+  T* da = static_cast<T*>(dav);
+  T* db = static_cast<T*>(dbv);
+  CHECK_HIP(hipMemcpyAsync(da, ha.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  if (invert) {
+    CHECK_HIP(launch_set_identity<T>(db, (int64_t)n, (int)n, ctx->stream), "identity");  // cholesky.rs:179
+  } else {
+    gather(b, brs, bcs, n, m, hb.data());
+    CHECK_HIP(hipMemcpyAsync(db, hb.data(), n * m * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  }
+  llh_status st = solve_on_device<T>(ctx, n, da, m, db, err);
+  if (st != LLH_OK) return st;
+  CHECK_HIP(hipMemcpyAsync(ha.data(), da, n * n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
+  CHECK_HIP(hipMemcpyAsync(hb.data(), db, n * m * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
+  CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
+            "status");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  const int32_t f = ctx->h_info[0];
+  if (f != kNoFail) {
+    const size_t fr = (size_t)f;
+    scatter(ha.data(), fr + 1, n, a, rs, cs, [fr](size_t i) { return i < fr ? i + 1 : fr; });
+    return not_pd(err, f);
+  }
+  scatter(ha.data(), n, n, a, rs, cs, [](size_t i) { return i + 1; });  // dirty factor
+  if (invert)
+    scatter(hb.data(), n, n, inv_out, ors, ocs, [n](size_t) { return n; });
+  else
+    scatter(hb.data(), n, m, b, brs, bcs, [m](size_t) { return m; });
+  return ok(err);
+}
+
+// ---- device-resident forms ------------------------------------------------
+
+template <typename T>
+llh_status dev_potrf(llh_context* ctx, size_t n, T* a, size_t lda, int zero_upper, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (n == 0) return ok(err);
+  if (a == nullptr || lda < n) return invalid(err, "null matrix or lda < n");
+  if (n > 0x7fffffff) return invalid(err, "matrix too large");
+  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, zero_upper != 0, ctx->stream), "potrf");
+  CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
+            "status");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  if (ctx->h_info[0] != kNoFail) return not_pd(err, ctx->h_info[0]);
+  return ok(err);
+}
+
+template <typename T>
+llh_status dev_solve_triangular(llh_context* ctx, llh_uplo uplo, int trans, size_t n, const T* a, size_t lda,
+                                size_t m, T* b, size_t ldb, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (n == 0 || m == 0) return ok(err);
+  if (a == nullptr || b == nullptr || lda < n || ldb < m) return invalid(err, "bad matrix arguments");
+  CHECK_HIP(trsm_enqueue<T>(uplo == LLH_UPPER, trans != 0, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb,
+                            ctx->stream),
+            "trsm");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  return ok(err);
+}
+
+template <typename T>
+llh_status dev_solvec(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T* b, size_t ldb,
+                      llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (n == 0) return ok(err);
+  if (a == nullptr || lda < n || (m > 0 && (b == nullptr || ldb < m))) return invalid(err, "bad arguments");
+  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, false, ctx->stream), "potrf");
+  CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
+            "status");
+  if (m > 0) {
+    // Solves run regardless of the status and are discarded on failure; this
+    // keeps one synchronisation per call.
+    CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, ctx->stream),
+              "forward solve");
+    CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, ctx->stream),
+              "back solve");
+  }
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  if (ctx->h_info[0] != kNoFail) return not_pd(err, ctx->h_info[0]);
+  return ok(err);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* llh_status_string(int32_t code) {
+  switch (code) {
+    case LLH_OK: return "ok";
+    case LLH_ERR_NOT_SQUARE: return "NotSquare";
+    case LLH_ERR_NOT_POSITIVE_DEFINITE: return "NotPositiveDefinite";
+    case LLH_ERR_WRONG_ROWS: return "WrongRows";
+    case LLH_ERR_INVALID_ARGUMENT: return "InvalidArgument";
+    case LLH_ERR_HIP: return "HipError";
+    case LLH_ERR_OUT_OF_MEMORY: return "OutOfMemory";
+    case LLH_ERR_NO_CONVERGENCE: return "NoConvergence";
+    case LLH_ERR_REJECTED: return "Rejected";
+    default: return "unknown";
+  }
+}
+
+llh_status llh_context_create(int device, llh_context** out, llh_error* err) {
+  if (out == nullptr) return invalid(err, "null output pointer");
+  *out = nullptr;
+  int dev = device;
+  if (dev < 0) CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
+  CHECK_HIP(hipSetDevice(dev), "hipSetDevice");
+  auto* ctx = new llh_context();
+  ctx->device = dev;
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&ctx->d_info, 64 * sizeof(int32_t));
+  if (e == hipSuccess) e = hipHostMalloc(&ctx->h_info, 64 * sizeof(int32_t), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    llh_context_destroy(ctx);
+    return hip_fail(err, e, "context create");
+  }
+  ctx->own_stream = true;
+  *out = ctx;
+  return ok(err);
+}
+
+void llh_context_destroy(llh_context* ctx) {
+  if (ctx == nullptr || ctx->magic != llh_context::kMagic) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& b : ctx->scratch)
+    if (b.ptr) (void)hipFree(b.ptr);
+  if (ctx->d_info) (void)hipFree(ctx->d_info);
+  if (ctx->h_info) (void)hipHostFree(ctx->h_info);
+  if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  ctx->magic = 0;
+  delete ctx;
+}
+
+llh_status llh_context_set_stream(llh_context* ctx, void* hip_stream, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  ctx->stream = static_cast<hipStream_t>(hip_stream);
+  ctx->own_stream = false;
+  return ok(err);
+}
+
+void* llh_context_stream(llh_context* ctx) { return bad_ctx(ctx) ? nullptr : (void*)ctx->stream; }
+
+#define LLH_PLAIN_IMPL(T, sfx)                                                                          \
+  llh_status llh_cholesky_inplace_dirty_##sfx(llh_context* ctx, size_t rows, size_t cols, T* a,         \
+                                              ptrdiff_t rs, ptrdiff_t cs, llh_error* err) {             \
+    return cholesky_plain<T>(ctx, rows, cols, a, rs, cs, false, err);                                   \
+  }                                                                                                     \
+  llh_status llh_cholesky_inplace_##sfx(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdiff_t rs,  \
+                                        ptrdiff_t cs, llh_error* err) {                                 \
+    return cholesky_plain<T>(ctx, rows, cols, a, rs, cs, true, err);                                    \
+  }                                                                                                     \
+  llh_status llh_triangular_inplace_##sfx(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdiff_t rs,\
+                                          ptrdiff_t cs, llh_uplo uplo, llh_error* err) {                \
+    return triangular_plain<T>(ctx, rows, cols, a, rs, cs, uplo, err);                                  \
+  }                                                                                                     \
+  llh_status llh_is_triangular_##sfx(llh_context* ctx, size_t rows, size_t cols, const T* a,            \
+                                     ptrdiff_t rs, ptrdiff_t cs, llh_uplo uplo, int* out,               \
+                                     llh_error* err) {                                                  \
+    return is_triangular_plain<T>(ctx, rows, cols, a, rs, cs, uplo, out, err);                          \
+  }                                                                                                     \
+  llh_status llh_solve_triangular_inplace_##sfx(llh_context* ctx, size_t rows, size_t cols, const T* a, \
+                                                ptrdiff_t rs, ptrdiff_t cs, size_t b_rows,              \
+                                                size_t b_cols, T* b, ptrdiff_t brs, ptrdiff_t bcs,      \
+                                                llh_uplo uplo, llh_error* err) {                        \
+    return solve_triangular_plain<T>(ctx, rows, cols, a, rs, cs, b_rows, b_cols, b, brs, bcs, uplo,     \
+                                     err);                                                              \
+  }                                                                                                     \
+  llh_status llh_solvec_inplace_##sfx(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdiff_t rs,    \
+                                      ptrdiff_t cs, size_t b_rows, size_t b_cols, T* b, ptrdiff_t brs,  \
+                                      ptrdiff_t bcs, llh_error* err) {                                  \
+    return solvec_plain<T>(ctx, rows, cols, a, rs, cs, b_rows, b_cols, b, brs, bcs, nullptr, 0, 0, err); \
+  }                                                                                                     \
+  llh_status llh_invc_inplace_##sfx(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdiff_t rs,      \
+                                    ptrdiff_t cs, T* out, ptrdiff_t ors, ptrdiff_t ocs, llh_error* err) { \
+    if (out == nullptr && rows > 0) return invalid(err, "null output pointer");                         \
+    if (rows == 0) {                                                                                    \
+      if (rows != cols) return not_square(err, rows, cols);                                             \
+      return ok(err);                                                                                   \
+    }                                                                                                   \
+    return solvec_plain<T>(ctx, rows, cols, a, rs, cs, rows, rows, nullptr, 0, 0, out, ors, ocs, err);   \
+  }                                                                                                     \
+  llh_status llh_dev_potrf_##sfx(llh_context* ctx, size_t n, T* a, size_t lda, int zero_upper,          \
+                                 llh_error* err) {                                                      \
+    return dev_potrf<T>(ctx, n, a, lda, zero_upper, err);                                               \
+  }                                                                                                     \
+  llh_status llh_dev_solve_triangular_##sfx(llh_context* ctx, llh_uplo uplo, int trans, size_t n,       \
+                                            const T* a, size_t lda, size_t m, T* b, size_t ldb,         \
+                                            llh_error* err) {                                           \
+    return dev_solve_triangular<T>(ctx, uplo, trans, n, a, lda, m, b, ldb, err);                        \
+  }                                                                                                     \
+  llh_status llh_dev_solvec_##sfx(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T* b,         \
+                                  size_t ldb, llh_error* err) {                                         \
+    return dev_solvec<T>(ctx, n, a, lda, m, b, ldb, err);                                               \
+  }
+
+LLH_PLAIN_IMPL(double, d)
+LLH_PLAIN_IMPL(float, s)
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Host drivers: the blocked right-looking POTRF and the blocked triangular
+// solves, as sequences of kernel launches on one stream.
+#include <algorithm>
+
+#include "llh_driver.h"
+#include "llh_kernels.h"
+
+namespace llh {
+
+#define LLH_TRY(expr)                  \
+  do {                                 \
+    hipError_t e_ = (expr);            \
+    if (e_ != hipSuccess) return e_;   \
+  } while (0)
+
+// Right-looking blocked Cholesky of the lower triangle (the blocked equivalent
+// of cholesky.rs:54-74).  For each kNBO-wide outer panel, kNB-wide inner steps
+// factor the diagonal block in LDS, solve the rows below it, and update the rest
+// of the outer panel; then one rank-kNBO SYRK on MFMA updates the trailing
+// matrix.  Matrices no larger than the LDS kernel's limit are factored by that
+// kernel alone, in the reference's exact operation order.
+template <typename T>
+hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, hipStream_t s) {
+  LLH_TRY(launch_fill_i32(info, kNoFail, 1, s));
+  if (n <= 0) return hipSuccess;
+  if (n <= small_max<T>()) {
+    LLH_TRY(launch_potrf_exact_single<T>(a, lda, n, info, 0, nullptr, s));
+  } else {
+    for (int k0 = 0; k0 < n; k0 += kNBO) {
+      const int kb = std::min(kNBO, n - k0);
+      for (int k = k0; k < k0 + kb; k += kNB) {
+        const int b = std::min(kNB, k0 + kb - k);
+        T* akk = a + (int64_t)k * lda + k;
+        LLH_TRY(launch_potrf_exact_single<T>(akk, lda, b, info, k, info, s));
+        const int below = n - k - b;
+        if (below <= 0) continue;
+        T* panel = akk + (int64_t)b * lda;  // rows k+b.., columns k..k+b
+        LLH_TRY(launch_trsm_rows<T>(below, b, panel, lda, akk, lda, info, s));
+        const int cols = k0 + kb - (k + b);
+        if (cols > 0)
+          LLH_TRY(launch_gemm<T>(false, true, true, below, cols, b, T(-1), panel, lda, panel, lda, T(1),
+                                 panel + b, lda, info, s));
+      }
+      const int m2 = n - k0 - kb;
+      if (m2 > 0) {
+        T* p = a + (int64_t)(k0 + kb) * lda + k0;
+        LLH_TRY(launch_gemm<T>(false, true, true, m2, m2, kb, T(-1), p, lda, p, lda, T(1), p + kb, lda, info,
+                               s));
+      }
+    }
+  }
+  if (zero_upper) LLH_TRY(launch_zero_triangle<T>(a, lda, n, true, info, s));
+  return hipSuccess;
+}
+
+// Blocked substitution for op(A) X = B (B row-major n x m, in place), kNB rows
+// at a time: solve the diagonal block, then subtract its contribution from the
+// rows still to come with one GEMM (the column sweep of triangular.rs:113-125,
+// done a block of rows at a time).
+template <typename T>
+hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t lda, int m, T* b, int64_t ldb,
+                        hipStream_t s) {
+  if (n <= 0 || m <= 0) return hipSuccess;
+  const int nblk = (n + kNB - 1) / kNB;
+  if (!upper_op) {  // forward
+    for (int bi = 0; bi < nblk; ++bi) {
+      const int i0 = bi * kNB, bs = std::min(kNB, n - i0);
+      const T* tkk = a + (int64_t)i0 * lda + i0;
+      LLH_TRY(launch_trsv_block<T>(false, trans, bs, tkk, lda, m, b + (int64_t)i0 * ldb, ldb, s));
+      const int rest = n - i0 - bs;
+      if (rest <= 0) continue;
+      // B[i0+bs:, :] -= op(A)[i0+bs:, i0:i0+bs] * X[i0:i0+bs, :]
+      const T* blk = trans ? a + (int64_t)i0 * lda + (i0 + bs) : a + (int64_t)(i0 + bs) * lda + i0;
+      LLH_TRY(launch_gemm<T>(trans, false, false, rest, m, bs, T(-1), blk, lda, b + (int64_t)i0 * ldb, ldb,
+                             T(1), b + (int64_t)(i0 + bs) * ldb, ldb, nullptr, s));
+    }
+  } else {  // backward
+    for (int bi = nblk - 1; bi >= 0; --bi) {
+      const int i0 = bi * kNB, bs = std::min(kNB, n - i0);
+      const T* tkk = a + (int64_t)i0 * lda + i0;
+      LLH_TRY(launch_trsv_block<T>(true, trans, bs, tkk, lda, m, b + (int64_t)i0 * ldb, ldb, s));
+      if (i0 == 0) continue;
+      // B[:i0, :] -= op(A)[:i0, i0:i0+bs] * X[i0:i0+bs, :]
+      const T* blk = trans ? a + (int64_t)i0 * lda : a + i0;
+      LLH_TRY(launch_gemm<T>(trans, false, false, i0, m, bs, T(-1), blk, lda, b + (int64_t)i0 * ldb, ldb,
+                             T(1), b, ldb, nullptr, s));
+    }
+  }
+  return hipSuccess;
+}
+
+template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool, hipStream_t);
+template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, hipStream_t);
+template hipError_t trsm_enqueue<double>(bool, bool, int, const double*, int64_t, int, double*, int64_t,
+                                         hipStream_t);
+template hipError_t trsm_enqueue<float>(bool, bool, int, const float*, int64_t, int, float*, int64_t,
+                                        hipStream_t);
+
+}  // namespace llh

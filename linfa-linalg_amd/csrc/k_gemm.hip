@@ -1,0 +1,226 @@
+// MFMA GEMM for the trailing SYRK update of the blocked POTRF and the block
+// updates of the triangular solves.
+//
+//   C = beta * C + alpha * op(A) * op(B)        (TRI: only entries with row >= col)
+//
+// op(A) is M x K: TA = false reads A[i*lda + k] (row-major M x K), TA = true
+// reads A[k*lda + i] (row-major K x M).  op(B) is K x N: TB = true reads
+// B[j*ldb + k] (row-major N x K, i.e. C = A * B^T as in SYRK), TB = false reads
+// B[k*ldb + j] (row-major K x N).
+//
+// Tiling for CDNA4: a BM x BN workgroup tile over WM x WN wave64 waves, each
+// wave accumulating (BM/WM) x (BN/WN) in 16x16 MFMA blocks
+// (v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32).  K advances in slabs of
+// 16 staged through LDS in k-major order, so a fragment read is 16 consecutive
+// elements of one LDS row; the next slab is prefetched into registers while the
+// current one is multiplied.  Lane layouts (measured on gfx950):
+//   A operand: lane l holds op(A)[l%16][k0 + l/16]
+//   B operand: lane l holds op(B)[k0 + l/16][l%16]
+//   f64 result reg i: C[4*i + l/16][l%16];  f32 result reg i: C[4*(l/16) + i][l%16]
+#include "llh_common.h"
+#include "llh_kernels.h"
+
+namespace llh {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc_t = v4d;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int i) { return 4 * i + (lane >> 4); }
+};
+template <>
+struct Mfma<float> {
+  using acc_t = v4f;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
+};
+
+constexpr int BK = 16;
+
+// Loads one BK-slab of an operand tile (R rows of op(X), i.e. R output rows or
+// columns) from global memory into registers.  KMAJOR_SRC: the source is stored
+// with k as its row index (contiguous along the R direction); otherwise the
+// source rows are the R index and k is contiguous.
+template <typename T, int R, int NTHR, bool KMAJOR_SRC>
+struct SlabLoader {
+  static constexpr int PER = (R * BK) / NTHR;  // elements per thread
+  static_assert((R * BK) % NTHR == 0, "tile must divide over the threads");
+  T v[PER];
+
+  __device__ __forceinline__ void load(const T* __restrict__ X, int64_t ldx, int r0, int rmax, int k0,
+                                       int kmax) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int rr, kk;
+      if constexpr (KMAJOR_SRC) {  // consecutive threads walk the contiguous r
+        const int e = t + p * NTHR;
+        rr = e % R;
+        kk = e / R;
+      } else {  // consecutive threads walk the contiguous k
+        const int e = t + p * NTHR;
+        kk = e % BK;
+        rr = e / BK;
+      }
+      const int gr = r0 + rr, gk = k0 + kk;
+      T x = T(0);
+      if (gr < rmax && gk < kmax) x = KMAJOR_SRC ? X[(int64_t)gk * ldx + gr] : X[(int64_t)gr * ldx + gk];
+      v[p] = x;
+    }
+  }
+  template <int LDR>
+  __device__ __forceinline__ void store(T* S) const {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int rr, kk;
+      const int e = t + p * NTHR;
+      if constexpr (KMAJOR_SRC) {
+        rr = e % R;
+        kk = e / R;
+      } else {
+        kk = e % BK;
+        rr = e / BK;
+      }
+      S[kk * LDR + rr] = v[p];
+    }
+  }
+};
+
+template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, bool TRI>
+__global__ __launch_bounds__(WM* WN * 64) void gemm_kernel(int M, int N, int K, T alpha,
+                                                           const T* __restrict__ A, int64_t lda,
+                                                           const T* __restrict__ B, int64_t ldb, T beta,
+                                                           T* __restrict__ C, int64_t ldc,
+                                                           const int32_t* __restrict__ gate) {
+  constexpr int NTHR = WM * WN * 64;
+  constexpr int TM = BM / WM, TN = BN / WN;
+  constexpr int FM = TM / 16, FN = TN / 16;
+  constexpr int PAD = sizeof(T) == 8 ? 2 : 4;
+  constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
+  using MF = Mfma<T>;
+  using acc_t = typename MF::acc_t;
+
+  if (gate != nullptr && *gate != kNoFail) return;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  if (TRI && m0 + BM - 1 < n0) return;  // tile entirely above the diagonal
+
+  __shared__ T As[BK * LDA_S];
+  __shared__ T Bs[BK * LDB_S];
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm0 = (wave / WN) * TM, wn0 = (wave % WN) * TN;
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  SlabLoader<T, BM, NTHR, TA> la;
+  SlabLoader<T, BN, NTHR, !TB> lb;
+  la.load(A, lda, m0, M, 0, K);
+  lb.load(B, ldb, n0, N, 0, K);
+  la.template store<LDA_S>(As);
+  lb.template store<LDB_S>(Bs);
+  __syncthreads();
+
+  const int fr = lane & 15, fk = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    const bool more = k0 + BK < K;
+    if (more) {
+      la.load(A, lda, m0, M, k0 + BK, K);
+      lb.load(B, ldb, n0, N, k0 + BK, K);
+    }
+#pragma unroll
+    for (int s = 0; s < BK / 4; ++s) {
+      T af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = As[(s * 4 + fk) * LDA_S + wm0 + i * 16 + fr];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = Bs[(s * 4 + fk) * LDB_S + wn0 + j * 16 + fr];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[i], bf[j], acc[i][j]);
+    }
+    __syncthreads();
+    if (more) {
+      la.template store<LDA_S>(As);
+      lb.template store<LDB_S>(Bs);
+      __syncthreads();
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = m0 + wm0 + i * 16 + MF::row(lane, r);
+        const int gj = n0 + wn0 + j * 16 + fr;
+        if (gi < M && gj < N && (!TRI || gi >= gj)) {
+          T* c = C + (int64_t)gi * ldc + gj;
+          const T v = alpha * acc[i][j][r];
+          *c = (beta == T(0)) ? v : beta * *c + v;
+        }
+      }
+}
+
+template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, bool TRI>
+hipError_t launch_cfg(int M, int N, int K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
+                      T beta, T* C, int64_t ldc, const int32_t* gate, hipStream_t s) {
+  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
+  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, WM, WN, TA, TB, TRI>), grid, dim3(WM * WN * 64), 0, s, M, N,
+                     K, alpha, A, lda, B, ldb, beta, C, ldc, gate);
+  return hipGetLastError();
+}
+
+template <typename T, bool TA, bool TB, bool TRI>
+hipError_t launch_shape(int M, int N, int K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
+                        T beta, T* C, int64_t ldc, const int32_t* gate, hipStream_t s) {
+  if (N <= 32)
+    return launch_cfg<T, 128, 32, 4, 1, TA, TB, TRI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
+  if (N <= 64)
+    return launch_cfg<T, 128, 64, 2, 2, TA, TB, TRI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
+  return launch_cfg<T, 128, 128, 2, 2, TA, TB, TRI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha, const T* A, int64_t lda,
+                       const T* B, int64_t ldb, T beta, T* C, int64_t ldc, const int32_t* gate,
+                       hipStream_t s) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (K <= 0) K = 0;
+  if (!ta && tb && tri)
+    return launch_shape<T, false, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
+  if (!ta && tb && !tri)
+    return launch_shape<T, false, true, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
+  if (!ta && !tb && !tri)
+    return launch_shape<T, false, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
+  if (ta && !tb && !tri)
+    return launch_shape<T, true, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
+  return hipErrorInvalidValue;
+}
+
+template hipError_t launch_gemm<double>(bool, bool, bool, int, int, int, double, const double*, int64_t,
+                                        const double*, int64_t, double, double*, int64_t, const int32_t*,
+                                        hipStream_t);
+template hipError_t launch_gemm<float>(bool, bool, bool, int, int, int, float, const float*, int64_t,
+                                       const float*, int64_t, float, float*, int64_t, const int32_t*,
+                                       hipStream_t);
+
+}  // namespace llh

@@ -1,0 +1,38 @@
+// Shared definitions for the linfa-linalg-hip kernels and host drivers (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace llh {
+
+// Status word written by the factor kernels: the first failing row of the
+// `d <= 0` pivot test (cholesky.rs:69), or kNoFail.
+constexpr int32_t kNoFail = 0x7fffffff;
+
+// Block sizes of the blocked right-looking POTRF: kNB is the width of the
+// diagonal block factored in LDS and of the panel solved row by row; kNBO is
+// the rank of the trailing SYRK update on MFMA.
+constexpr int kNB = 64;
+constexpr int kNBO = 256;
+
+// Largest n the single-workgroup LDS kernel takes (the packed lower triangle of
+// an f64 matrix must fit in 160 KiB of LDS with room to spare).
+constexpr int kSmallMaxD = 128;
+constexpr int kSmallMaxS = 176;
+
+// Exact-order arithmetic.  The files that use these helpers are compiled with
+// -ffp-contract=off and open with `#pragma clang fp contract(off)`, so every
+// `s += x*y` stays a separately rounded multiply and add, as the reference's
+// compiled Rust is.  IEEE division and square root are correctly rounded on
+// gfx950 for f64 and, without fast-math flags, for f32.
+template <typename T>
+__device__ __forceinline__ T sqrt_rn(T a);
+template <>
+__device__ __forceinline__ double sqrt_rn<double>(double a) { return __builtin_sqrt(a); }
+template <>
+__device__ __forceinline__ float sqrt_rn<float>(float a) { return __builtin_sqrtf(a); }
+
+}  // namespace llh

@@ -1,0 +1,35 @@
+// The context behind the opaque llh_context handle of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "llh.h"
+
+struct llh_context {
+  static constexpr uint32_t kMagic = 0x4c4c4843;  // "LLHC"
+  uint32_t magic = kMagic;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int32_t* d_info = nullptr;  // device status words: [0] factor status, [1] flag
+  int32_t* h_info = nullptr;  // pinned host mirror
+  struct Buffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+  };
+  Buffer scratch[4];  // grow-only device buffers of the plain forms
+};
+
+namespace llh {
+llh_status set_error(llh_error* err, llh_status code, uint64_t a, uint64_t b, int64_t column, const char* msg,
+                     int hip_error);
+llh_status ok(llh_error* err);
+llh_status hip_fail(llh_error* err, hipError_t e, const char* where);
+llh_status invalid(llh_error* err, const char* msg);
+llh_status not_square(llh_error* err, size_t rows, size_t cols);
+llh_status not_pd(llh_error* err, int64_t col);
+llh_status wrong_rows(llh_error* err, size_t expected, size_t actual);
+hipError_t ctx_buffer(llh_context* ctx, int slot, size_t bytes, void** out);
+}  // namespace llh

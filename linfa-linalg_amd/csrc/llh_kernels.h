@@ -1,0 +1,62 @@
+// Kernel launchers shared by the host drivers.  Every launcher enqueues on the
+// given stream and returns the launch status; `gate`, when non-null, points at
+// the factor's status word and makes the kernel a no-op once a pivot has failed.
+#pragma once
+
+#include "llh_common.h"
+
+namespace llh {
+
+// One matrix of a batched small factor/solve: A row-major n x n (lda), B
+// row-major n x m (ldb), info receives kNoFail or the first failing row.
+template <typename T>
+struct SmallDesc {
+  T* a;
+  T* b;
+  int64_t lda;
+  int64_t ldb;
+  int32_t n;
+  int32_t m;
+  int32_t* info;
+};
+
+template <typename T>
+constexpr int small_max() { return sizeof(T) == 8 ? kSmallMaxD : kSmallMaxS; }
+template <typename T>
+struct small_max_c { static constexpr int value = small_max<T>(); };
+
+// k_small.hip: exact-order LDS Cholesky (bit-identical to the reference order).
+template <typename T>
+hipError_t launch_potrf_exact_single(T* a, int64_t lda, int n, int32_t* info, int64_t info_base,
+                                     const int32_t* gate, hipStream_t s);
+template <typename T>
+hipError_t launch_potrf_potrs_exact_batched(const SmallDesc<T>* descs, int count, int nmax, int do_solve,
+                                            hipStream_t s);
+
+// k_gemm.hip: C = beta*C + alpha*op(A)*op(B), optionally lower-triangle only.
+template <typename T>
+hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha, const T* A, int64_t lda,
+                       const T* B, int64_t ldb, T beta, T* C, int64_t ldc, const int32_t* gate,
+                       hipStream_t s);
+
+// k_trsm.hip: panel solve X := X * L^-T (X is m x b, L is b x b lower, b <= kNB).
+template <typename T>
+hipError_t launch_trsm_rows(int m, int b, T* X, int64_t ldx, const T* L, int64_t ldl, const int32_t* gate,
+                            hipStream_t s);
+// k_trsm.hip: diagonal-block solve op(T) Y = Y for b <= kNB rows and m columns
+// of a row-major Y; upper_op selects back substitution; trans reads T^T.
+template <typename T>
+hipError_t launch_trsv_block(bool upper_op, bool trans, int b, const T* Tm, int64_t ldt, int m, T* Y,
+                             int64_t ldy, hipStream_t s);
+
+// k_util.hip
+hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t count, hipStream_t s);
+template <typename T>
+hipError_t launch_zero_triangle(T* a, int64_t lda, int n, bool zero_upper, const int32_t* gate,
+                                hipStream_t s);
+template <typename T>
+hipError_t launch_is_triangular(const T* a, int64_t lda, int n, bool upper, int32_t* flag, hipStream_t s);
+template <typename T>
+hipError_t launch_set_identity(T* a, int64_t lda, int n, hipStream_t s);
+
+}  // namespace llh

@@ -1,0 +1,77 @@
+"""ctypes bindings of include/llh.h (the C ABI of libllh.so)."""
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libllh.so")
+HEADER = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include", "llh.h")
+
+STATUS = {0: "ok", 1: "NotSquare", 2: "NotPositiveDefinite", 3: "WrongRows",
+          16: "InvalidArgument", 17: "HipError", 18: "OutOfMemory", 19: "NoConvergence",
+          20: "Rejected"}
+
+
+def status_name(code):
+    return STATUS.get(code, f"status {code}")
+
+
+class LlhError(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int32), ("hip_error", ctypes.c_int32), ("a", ctypes.c_uint64),
+                ("b", ctypes.c_uint64), ("column", ctypes.c_int64), ("message", ctypes.c_char * 192)]
+
+
+_lib = None
+
+_sz, _pd, _vp, _int = ctypes.c_size_t, ctypes.c_ssize_t, ctypes.c_void_p, ctypes.c_int
+_err = ctypes.POINTER(LlhError)
+
+
+def _declare(lib):
+    lib.llh_context_create.argtypes = [_int, ctypes.POINTER(ctypes.c_void_p), _err]
+    lib.llh_context_destroy.argtypes = [_vp]
+    lib.llh_context_destroy.restype = None
+    lib.llh_context_set_stream.argtypes = [_vp, _vp, _err]
+    lib.llh_context_stream.argtypes = [_vp]
+    lib.llh_context_stream.restype = _vp
+    lib.llh_status_string.argtypes = [ctypes.c_int32]
+    lib.llh_status_string.restype = ctypes.c_char_p
+    for s in ("d", "s"):
+        f = lambda n: getattr(lib, f"llh_{n}_{s}")  # noqa: E731
+        f("cholesky_inplace_dirty").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _err]
+        f("cholesky_inplace").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _err]
+        f("triangular_inplace").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _int, _err]
+        f("is_triangular").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _int, ctypes.POINTER(_int), _err]
+        f("solve_triangular_inplace").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _sz, _sz, _vp, _pd, _pd,
+                                                  _int, _err]
+        f("solvec_inplace").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _sz, _sz, _vp, _pd, _pd, _err]
+        f("invc_inplace").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _vp, _pd, _pd, _err]
+        f("dev_potrf").argtypes = [_vp, _sz, _vp, _sz, _int, _err]
+        f("dev_solve_triangular").argtypes = [_vp, _int, _int, _sz, _vp, _sz, _sz, _vp, _sz, _err]
+        f("dev_solvec").argtypes = [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _err]
+
+
+def lib():
+    """Loads libllh.so; raises if it has not been built (no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C linfa-linalg_amd` "
+                               "or __graft_entry__.build()")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+        try:
+            from . import _abi_ext
+            _abi_ext.declare(_lib)
+        except ImportError:
+            pass
+    return _lib
+
+
+def declared_symbols(header=HEADER):
+    """Every function name a header declares, macro families expanded by the C
+    preprocessor."""
+    import subprocess
+    out = subprocess.run(["gcc", "-E", "-P", "-x", "c", header], check=True, capture_output=True,
+                         text=True).stdout
+    return sorted(set(re.findall(r"\b(llh_[a-z0-9_]+)\s*\(", out)))

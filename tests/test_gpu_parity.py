@@ -1,0 +1,257 @@
+"""Parity of the HIP path with the oracle (and, through it, the reference).
+
+- every known-answer and exact-integer pin (tests/pins.py) through the host
+  mirror, which calls the C ABI;
+- bit-identical factors and solutions for matrices the exact-order LDS kernel
+  takes (n <= 128 f64, n <= 176 f32), against the oracle and against the
+  committed seeded fixtures;
+- the blocked path at larger n: backward error ||A - L L^T||_F / ||A||_F
+  <= 1e-13 (f64) and solution error against the oracle's solution;
+- the reference's property tests with random strided layouts;
+- error paths, including the partially written array after NotPositiveDefinite.
+"""
+import os
+
+import numpy as np
+import pytest
+from hypothesis import given, settings, strategies as st
+
+import linfa_linalg_hip as llh
+from oracle import oracle as orc
+from pins import pin_failures, integer_factor
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+U, L = llh.UPLO.Upper, llh.UPLO.Lower
+
+
+def spd(n, seed, dtype=np.float64, shift=1.0):
+    """A = B B^T / n + shift I with B uniform in [-1, 1] (the C1 construction)."""
+    rng = np.random.default_rng(seed)
+    b = rng.uniform(-1.0, 1.0, size=(n, n))
+    return (b @ b.T / n + shift * np.eye(n)).astype(dtype)
+
+
+def backward_error(a, l):
+    l = np.tril(l).astype(np.float64)
+    a = a.astype(np.float64)
+    al = np.tril(a)
+    r = np.tril(l @ l.T) - al
+    return np.linalg.norm(r) / np.linalg.norm(al)
+
+
+def test_pins_through_hip():
+    assert pin_failures(llh) == []
+
+
+@pytest.mark.parametrize("dtype,nmax", [(np.float64, 128), (np.float32, 176)])
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 16, 33, 64, 65, 100, 128, 176])
+def test_small_bit_identical(n, dtype, nmax):
+    if n > nmax:
+        pytest.skip("above the exact kernel's size")
+    a = spd(n, 10 + n, dtype)
+    assert np.array_equal(llh.cholesky_dirty(a), orc.cholesky_dirty(a))
+    assert np.array_equal(llh.cholesky(a), orc.cholesky(a))
+    b = np.random.default_rng(n).standard_normal((n, 5)).astype(dtype)
+    a1, a2 = np.array(a), np.array(a)
+    x = llh.solvec(a1, b)
+    x_ref = orc.solvec(a2, b)
+    assert np.array_equal(x, x_ref)
+    assert np.array_equal(a1, a2)  # both left holding the dirty factor
+
+
+def test_seeded_fixtures_bit_identical():
+    fx = np.load(os.path.join(HERE, "golden", "seeded_oracle.npz"))
+    for tag in ("d", "s"):
+        for n in (1, 2, 3, 5, 8, 10, 16, 64, 128):
+            a, b = fx[f"{tag}{n}_a"], fx[f"{tag}{n}_b"]
+            fac = np.array(a)
+            x = llh.solvec(fac, b)
+            assert np.array_equal(fac, fx[f"{tag}{n}_dirty"]), (tag, n)
+            assert np.array_equal(x, fx[f"{tag}{n}_x"]), (tag, n)
+
+
+@pytest.mark.parametrize("n", [129, 200, 256, 300, 512, 777, 1024])
+def test_blocked_f64_against_oracle(n):
+    a = spd(n, n)
+    l_ref = orc.cholesky(a)
+    l = llh.cholesky(a)
+    assert backward_error(a, l) <= 1e-13
+    # both are backward stable factors of the same A: they agree to ~cond * eps
+    assert np.abs(l - l_ref).max() <= 1e-12 * np.abs(l_ref).max()
+    assert np.all(np.triu(l, 1) == 0)
+    rng = np.random.default_rng(n)
+    b = rng.standard_normal((n, 3))
+    x = llh.solvec(np.array(a), b)
+    x_ref = orc.solvec(np.array(a), b)
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-12
+
+
+def test_c1_config_n512():
+    """n=512 f64, A = B B^T/n + I (B uniform [-1,1]), cholesky_into then solvec
+    with one N(0,1) right-hand side; oracle = the reference algorithm in FP64."""
+    a = spd(512, 2024)
+    l = llh.cholesky_into(np.array(a))
+    assert backward_error(a, l) <= 1e-13
+    b = np.random.default_rng(7).standard_normal((512, 1))
+    x = llh.solvec(np.array(a), b)
+    x_ref = orc.solvec(np.array(a), b)
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-13
+
+
+@pytest.mark.parametrize("n", [200, 700])
+def test_blocked_f32(n):
+    a = spd(n, n, np.float32)
+    l = llh.cholesky(a)
+    assert backward_error(a, l) <= 1e-5
+    l_ref = orc.cholesky(a)
+    assert np.abs(l - l_ref).max() <= 1e-4 * np.abs(l_ref).max()
+
+
+@pytest.mark.parametrize("n", [5, 64, 65, 130, 333])
+@pytest.mark.parametrize("uplo", [U, L])
+@pytest.mark.parametrize("m", [1, 7, 64])
+def test_solve_triangular_against_oracle(n, uplo, m):
+    rng = np.random.default_rng(n * 10 + m)
+    t = rng.uniform(-1, 1, (n, n)) / np.sqrt(n)
+    t[np.diag_indices(n)] = rng.uniform(1, 2, n)
+    t = np.triu(t) if uplo == U else np.tril(t)
+    # NaN in the other triangle must never be read
+    other = np.triu(np.ones((n, n)), 1) if uplo == L else np.tril(np.ones((n, n)), -1)
+    tg = t + np.where(other > 0, np.nan, 0.0)
+    b = rng.standard_normal((n, m))
+    x = llh.solve_triangular(tg, b, uplo)
+    x_ref = orc.solve_triangular(t, b, uplo)
+    assert np.abs(x - x_ref).max() <= 1e-12 * max(1.0, np.abs(x_ref).max())
+
+
+@pytest.mark.parametrize("n", [3, 50, 150, 300])
+def test_invc(n):
+    a = spd(n, 3 * n)
+    inv = llh.invc(a)
+    assert np.abs(a @ inv - np.eye(n)).max() <= 1e-12
+    inv_ref = orc.invc(a)
+    assert np.abs(inv - inv_ref).max() <= 1e-12 * np.abs(inv_ref).max()
+
+
+# --- error paths ------------------------------------------------------------------
+
+@pytest.mark.parametrize("n,j", [(12, 7), (100, 64), (300, 5), (300, 200), (700, 511)])
+def test_not_pd_partial_state_matches_reference(n, j):
+    rng = np.random.default_rng(n + j)
+    a = spd(n, n + j)
+    a[j, j] = -1.0
+    work, ref = np.array(a), np.array(a)
+    with pytest.raises(llh.NotPositiveDefinite) as ei:
+        llh.cholesky_inplace_dirty(work)
+    assert ei.value.column == j
+    with pytest.raises(orc.OracleError):
+        orc.cholesky_inplace_dirty(ref)
+    tol = 0 if n <= 128 else 1e-12
+    assert np.abs(work[:j + 1] - ref[:j + 1]).max() <= tol
+    assert work[j, j] == a[j, j]
+    assert np.array_equal(work[j + 1:], a[j + 1:])
+    # solvec leaves the same dirty state and reports the same error
+    w2 = np.array(a)
+    with pytest.raises(llh.NotPositiveDefinite):
+        llh.solvec(w2, rng.standard_normal((n, 2)))
+    assert np.array_equal(w2, work)
+    del rng
+
+
+def test_error_precedence():
+    with pytest.raises(llh.NotSquare) as ei:
+        llh.solvec(np.zeros((2, 3)), np.zeros((2, 1)))
+    assert (ei.value.rows, ei.value.cols) == (2, 3)
+    a = spd(4, 1)
+    work = np.array(a)
+    with pytest.raises(llh.WrongRows) as ei:
+        llh.solvec_inplace(work, np.zeros((3, 1)))
+    assert (ei.value.expected, ei.value.actual) == (4, 3)
+    # the factor ran before WrongRows (cholesky.rs:140-141)
+    assert np.array_equal(work, orc.cholesky_dirty(a))
+    with pytest.raises(llh.NotPositiveDefinite):
+        llh.solvec(np.array([[1.0, 2.0], [2.0, 1.0]]), np.zeros((5, 1)))
+
+
+def test_nan_pivot_passes():
+    a = np.array([[4.0, 2.0], [2.0, np.nan]])
+    l = llh.cholesky(a)
+    assert l[0, 0] == 2.0 and l[1, 0] == 1.0 and np.isnan(l[1, 1])
+
+
+def test_zero_columns_and_empty():
+    a = spd(5, 1)
+    out = llh.solvec(np.array(a), np.zeros((5, 0)))
+    assert out.shape == (5, 0)
+    assert llh.cholesky(np.zeros((0, 0))).shape == (0, 0)
+    with pytest.raises(llh.WrongRows):
+        llh.solve_triangular(np.zeros((0, 0)), np.zeros((2, 2)), U)
+
+
+# --- the reference's property tests with strided layouts ------------------------------
+
+def layouts(a, flags):
+    ir, ic, tr = flags
+    if ir:
+        a = a[::-1, :]
+    if ic:
+        a = a[:, ::-1]
+    if tr and a.shape[0] == a.shape[1]:
+        a = a.T
+    return a
+
+
+@st.composite
+def matrix(draw, rows, cols):
+    data = draw(st.lists(st.floats(-100.0, 100.0), min_size=rows * cols, max_size=rows * cols))
+    return layouts(np.array(data).reshape(rows, cols), draw(st.tuples(st.booleans(), st.booleans(),
+                                                                        st.booleans())))
+
+
+@st.composite
+def hpd_system(draw):
+    n = draw(st.integers(1, 10))
+    a = draw(matrix(n, n))
+    m = a.T @ a
+    m[np.diag_indices(n)] += 1.0
+    return m, draw(matrix(n, draw(st.integers(1, 10))))
+
+
+@settings(max_examples=60, deadline=None)
+@given(hpd_system())
+def test_property_cholesky_solvec_invc(system):
+    """tests/cholesky.rs:21-67, through strided views; bit-identical here since
+    every n is within the exact kernel's range."""
+    a, x = system
+    chol = llh.cholesky(a)
+    assert np.abs(chol @ chol.T - a).max() <= 1e-7
+    assert llh.is_triangular(chol, L)
+    assert np.array_equal(chol, orc.cholesky(a))
+    b = a @ x
+    assert np.abs(llh.solvec(np.array(a), b) - x).max() <= 1e-5
+    # in place through a negative-stride view of b
+    bb = np.array(b)[::-1][::-1]
+    aa = np.array(a)[:, ::-1][:, ::-1]
+    llh.solvec_inplace(aa, bb)
+    assert np.array_equal(bb, orc.solvec(np.array(a), b))
+    assert np.abs(a @ llh.invc(a) - np.eye(a.shape[0])).max() <= 1e-7
+
+
+@settings(max_examples=60, deadline=None)
+@given(st.sampled_from([U, L]), st.integers(1, 10), st.integers(1, 10), st.data())
+def test_property_solve_triangular(uplo, n, m, data):
+    """tests/triangular.rs:9-47."""
+    a = data.draw(matrix(n, n))  # a strided view (inverted axes / transposed)
+    # into_triangular on the view, then |diag| < 1 -> 1 (tests/triangular.rs:10-18)
+    other = np.tril(np.ones((n, n)), -1) if uplo == U else np.triu(np.ones((n, n)), 1)
+    a[other > 0] = 0.0
+    d = np.diagonal(a).copy()
+    d[np.abs(d) < 1.0] = 1.0
+    a[np.diag_indices(n)] = d
+    x = data.draw(matrix(n, m))
+    b = a @ x
+    out = llh.solve_triangular(a, b, uplo)
+    assert np.abs(out - x).max() <= 1e-4
+    out2 = llh.solve_triangular_inplace(a, np.array(b), uplo)
+    assert np.abs(out2 - x).max() <= 1e-4
