@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Benchmark: SPD solve GFLOP/s at n = 32768 on one MI355X.
+
+One step = one SPD solve on device-resident inputs: the blocked POTRF of an
+n x n f64 matrix followed by the forward and back triangular solves with m
+right-hand sides (llh_dev_solvec_d, the device-memory form of solvec).  Inputs
+are synthetic and seeded: A = B B^T / n + lambda I with B ~ N(0, 1), built on
+the GPU before the timed region; every step factors its own copy of A, so the
+timed region contains only the solve.
+
+GFLOP/s counts the algorithmic work n^3/3 + n^2/2 + n/6 + 2 n^2 m.  The
+dominant kernel (the rank-256 trailing SYRK on MFMA) is timed live with HIP
+events on the launch stream; its roofline fraction is against the FP64 peak.
+The CPU baseline is the oracle (the reference algorithm restated in C, one
+thread) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "linfa-linalg_amd"))
+
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+PEAK_HBM_GBS = 8000.0
+
+
+def solve_flops(n, m):
+    return n ** 3 / 3.0 + n ** 2 / 2.0 + n / 6.0 + 2.0 * n * n * m
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, m, seed):
+    """The oracle (reference algorithm, C, one thread) on one solvec of size n."""
+    import numpy as np
+    from oracle import oracle as orc
+    rng = np.random.default_rng(seed)
+    b = rng.standard_normal((n, n))
+    a = b @ b.T / n + np.eye(n)
+    rhs = rng.standard_normal((n, m))
+    times = []
+    for _ in range(2):
+        work = np.array(a)
+        t0 = time.perf_counter()
+        orc.solvec(work, rhs)
+        times.append(time.perf_counter() - t0)
+    t = min(times)
+    return {"value": round(solve_flops(n, m) / t / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+            "sample": f"oracle solvec (reference algorithm in C, 1 thread), n={n}, {m} RHS, "
+                      f"A=B*B^T/n+I, best of 2: {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--lam", type=float, default=1.0)
+    ap.add_argument("--seed", type=int, default=20241202)
+    ap.add_argument("--cpu-n", type=int, default=2048)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU layout is described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from linfa_linalg_hip import device as dev
+
+    n, m = args.n, args.nrhs
+    torch.cuda.set_device(0)
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    log(f"bench: generating A = B B^T/n + {args.lam} I, n={n}, f64")
+    bmat = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    a0 = torch.matmul(bmat, bmat.T).div_(n)
+    del bmat
+    a0.diagonal().add_(args.lam)
+    rhs0 = torch.randn(n, m, dtype=torch.float64, device="cuda", generator=g)
+    copies = [(a0.clone(), rhs0.clone()) for _ in range(args.warmup + args.steps)]
+    torch.cuda.synchronize()
+
+    for i in range(args.warmup):
+        log(f"bench: warmup step {i}")
+        dev.solvec(*copies[i])
+    torch.cuda.synchronize()
+
+    dev.set_profiling(True)
+    torch.cuda.synchronize()
+    log(f"bench: timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        dev.solvec(*copies[args.warmup + i])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stats = dev.kernel_stats()
+    dev.set_profiling(False)
+    ms = (t1 - t0) * 1e3 / args.steps
+    gflops = solve_flops(n, m) / (ms / 1e3) / 1e9
+    log(f"bench: {ms:.2f} ms/step, {gflops:.1f} GFLOP/s")
+
+    kernels = {}
+    for name, (cnt, tot_ms, fl, by) in stats.items():
+        if cnt == 0:
+            continue
+        kernels[name] = {"launches_per_step": cnt // args.steps, "ms_per_step": round(tot_ms / args.steps, 3),
+                         "avg_launch_us": round(tot_ms / cnt * 1e3, 2),
+                         "tflops": round(fl / (tot_ms / 1e3) / 1e12, 2) if tot_ms > 0 else None,
+                         "gbs": round(by / (tot_ms / 1e3) / 1e9, 1) if tot_ms > 0 else None}
+    cnt, tot_ms, fl, by = stats["trailing_syrk"]
+    achieved = fl / (tot_ms / 1e3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_TFLOPS["f64"], 4), "traffic": None,
+                "kernel": "gemm_kernel<double,128,128,2,2,TA=0,TB=1,TRI=1> (trailing SYRK, rank 256)",
+                "avg_launch_us": round(tot_ms / cnt * 1e3, 2), "launches_per_step": cnt // args.steps,
+                "algorithmic_tflop_per_step": round(fl / args.steps / 1e12, 4)}
+
+    check = {}
+    if not args.no_check:
+        a_fac, x = copies[-1]
+        l = torch.tril(a_fac)
+        res = rhs0 - a0 @ x
+        check["residual"] = float(res.norm() / (a0.norm() * x.norm()))
+        al = torch.tril(a0)
+        check["backward_error"] = float(torch.tril(l @ l.T - a0).norm() / al.norm())
+        log(f"bench: residual {check['residual']:.3e} backward error {check['backward_error']:.3e}")
+        del l, res, al
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"bench: CPU baseline sample n={args.cpu_n}")
+        cpu = cpu_baseline(args.cpu_n, m, args.seed)
+
+    out = {
+        "metric": "SPD solve GFLOP/s at n=32768 (f64 POTRF + forward/back solve, 16 RHS)",
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded N(0,1) B, A = B B^T/n + lambda I, built on device)",
+        "config": {"workload": f"spd_solve n={n} nrhs={m} lambda={args.lam} f64 (llh_dev_solvec_d)",
+                   "n": n, "nrhs": m, "lambda": args.lam, "parallelism": "1 GPU"},
+        "solve_roofline_frac_f64": round(gflops / 1e3 / PEAK_TFLOPS["f64"], 4),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "check": check,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

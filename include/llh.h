@@ -74,6 +74,31 @@ llh_status llh_context_set_stream(llh_context* ctx, void* hip_stream, llh_error*
 void* llh_context_stream(llh_context* ctx);
 const char* llh_status_string(int32_t code);
 
+/* Live kernel timing (no counterpart in the reference; for benchmarks).  While
+ * enabled, the device-resident drivers bracket each launch of a kernel class
+ * with HIP events on the context's stream and add the launch's algorithmic
+ * flops and bytes.  Enabling clears earlier records. */
+typedef enum llh_kernel_class {
+  LLH_KC_TRAILING_SYRK = 0, /* rank-256 trailing update of the POTRF (MFMA) */
+  LLH_KC_PANEL_UPDATE = 1,  /* update inside the 256-wide outer panel (MFMA) */
+  LLH_KC_DIAG_FACTOR = 2,   /* 64 x 64 diagonal-block factor in LDS */
+  LLH_KC_PANEL_TRSM = 3,    /* panel row solve X := X L11^-T */
+  LLH_KC_SOLVE_GEMM = 4,    /* block update of the triangular solves (MFMA) */
+  LLH_KC_SOLVE_DIAG = 5     /* diagonal-block solve of the triangular solves */
+} llh_kernel_class;
+
+typedef struct llh_kernel_stats {
+  uint64_t launches;
+  double total_ms; /* sum of event-timed launch durations */
+  double flops;    /* sum of algorithmic flops */
+  double bytes;    /* sum of algorithmic bytes */
+} llh_kernel_stats;
+
+llh_status llh_context_set_profiling(llh_context* ctx, int enable, llh_error* err);
+/* Synchronizes the context's stream, then sums the records of one class. */
+llh_status llh_context_kernel_stats(llh_context* ctx, int kernel_class, llh_kernel_stats* out,
+                                    llh_error* err);
+
 /* ---------------------------------------------------------------------------
  * Plain forms: host memory, any strides.  _d = f64, _s = f32.
  * ------------------------------------------------------------------------- */

@@ -341,7 +341,9 @@ llh_status dev_potrf(llh_context* ctx, size_t n, T* a, size_t lda, int zero_uppe
   if (n == 0) return ok(err);
   if (a == nullptr || lda < n) return invalid(err, "null matrix or lda < n");
   if (n > 0x7fffffff) return invalid(err, "matrix too large");
-  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, zero_upper != 0, ctx->stream), "potrf");
+  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, zero_upper != 0, ctx->stream,
+                            ctx->profiler()),
+            "potrf");
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
             "status");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
@@ -356,7 +358,7 @@ llh_status dev_solve_triangular(llh_context* ctx, llh_uplo uplo, int trans, size
   if (n == 0 || m == 0) return ok(err);
   if (a == nullptr || b == nullptr || lda < n || ldb < m) return invalid(err, "bad matrix arguments");
   CHECK_HIP(trsm_enqueue<T>(uplo == LLH_UPPER, trans != 0, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb,
-                            ctx->stream),
+                            ctx->stream, ctx->profiler()),
             "trsm");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
   return ok(err);
@@ -368,15 +370,18 @@ llh_status dev_solvec(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T*
   if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
   if (n == 0) return ok(err);
   if (a == nullptr || lda < n || (m > 0 && (b == nullptr || ldb < m))) return invalid(err, "bad arguments");
-  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, false, ctx->stream), "potrf");
+  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, false, ctx->stream, ctx->profiler()),
+            "potrf");
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
             "status");
   if (m > 0) {
     // Solves run regardless of the status and are discarded on failure; this
     // keeps one synchronisation per call.
-    CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, ctx->stream),
+    CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, ctx->stream,
+                              ctx->profiler()),
               "forward solve");
-    CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, ctx->stream),
+    CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, ctx->stream,
+                              ctx->profiler()),
               "back solve");
   }
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
@@ -445,6 +450,37 @@ llh_status llh_context_set_stream(llh_context* ctx, void* hip_stream, llh_error*
 }
 
 void* llh_context_stream(llh_context* ctx) { return bad_ctx(ctx) ? nullptr : (void*)ctx->stream; }
+
+llh_status llh_context_set_profiling(llh_context* ctx, int enable, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  for (auto& r : ctx->prof.recs) {
+    ctx->prof.pool.push_back(r.start);
+    ctx->prof.pool.push_back(r.stop);
+  }
+  ctx->prof.recs.clear();
+  ctx->prof.on = enable != 0;
+  return ok(err);
+}
+
+llh_status llh_context_kernel_stats(llh_context* ctx, int kernel_class, llh_kernel_stats* out,
+                                    llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (out == nullptr || kernel_class < 0 || kernel_class >= kNumClasses) return invalid(err, "bad arguments");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  llh_kernel_stats st{0, 0.0, 0.0, 0.0};
+  for (auto& r : ctx->prof.recs) {
+    if (r.cls != kernel_class) continue;
+    float ms = 0.f;
+    CHECK_HIP(hipEventElapsedTime(&ms, r.start, r.stop), "hipEventElapsedTime");
+    st.launches += 1;
+    st.total_ms += ms;
+    st.flops += r.flops;
+    st.bytes += r.bytes;
+  }
+  *out = st;
+  return ok(err);
+}
 
 #define LLH_PLAIN_IMPL(T, sfx)                                                                          \
   llh_status llh_cholesky_inplace_dirty_##sfx(llh_context* ctx, size_t rows, size_t cols, T* a,         \

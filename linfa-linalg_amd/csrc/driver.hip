@@ -13,6 +13,14 @@ namespace llh {
     if (e_ != hipSuccess) return e_;   \
   } while (0)
 
+// Brackets one launch with the profiler's events (no-op when profiling is off).
+#define LLH_PROF(cls, flops, bytes, expr)                                       \
+  do {                                                                          \
+    const int pi_ = prof ? prof->begin((cls), (flops), (bytes), s) : -1;        \
+    LLH_TRY(expr);                                                              \
+    if (prof) prof->end(pi_, s);                                                \
+  } while (0)
+
 // Right-looking blocked Cholesky of the lower triangle (the blocked equivalent
 // of cholesky.rs:54-74).  For each kNBO-wide outer panel, kNB-wide inner steps
 // factor the diagonal block in LDS, solve the rows below it, and update the rest
@@ -20,7 +28,9 @@ namespace llh {
 // matrix.  Matrices no larger than the LDS kernel's limit are factored by that
 // kernel alone, in the reference's exact operation order.
 template <typename T>
-hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, hipStream_t s) {
+hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, hipStream_t s,
+                         Profiler* prof) {
+  const double es = (double)sizeof(T);
   LLH_TRY(launch_fill_i32(info, kNoFail, 1, s));
   if (n <= 0) return hipSuccess;
   if (n <= small_max<T>()) {
@@ -31,21 +41,33 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
       for (int k = k0; k < k0 + kb; k += kNB) {
         const int b = std::min(kNB, k0 + kb - k);
         T* akk = a + (int64_t)k * lda + k;
-        LLH_TRY(launch_potrf_exact_single<T>(akk, lda, b, info, k, info, s));
+        const double db = b;
+        LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * (db + 1.0),
+                 (launch_potrf_exact_single<T>(akk, lda, b, info, k, info, s)));
         const int below = n - k - b;
         if (below <= 0) continue;
         T* panel = akk + (int64_t)b * lda;  // rows k+b.., columns k..k+b
-        LLH_TRY(launch_trsm_rows<T>(below, b, panel, lda, akk, lda, info, s));
+        LLH_PROF(kPanelTrsm, (double)below * db * db, es * ((double)below * db * 2.0 + db * db / 2.0),
+                 (launch_trsm_rows<T>(below, b, panel, lda, akk, lda, info, s)));
         const int cols = k0 + kb - (k + b);
-        if (cols > 0)
-          LLH_TRY(launch_gemm<T>(false, true, true, below, cols, b, T(-1), panel, lda, panel, lda, T(1),
-                                 panel + b, lda, info, s));
+        if (cols > 0) {
+          // rows below x cols, lower part of the top cols x cols square only
+          const double outs = (double)below * cols - (double)cols * (cols - 1) / 2.0;
+          LLH_PROF(kPanelUpdate, 2.0 * outs * db, es * (2.0 * outs + ((double)below + cols) * db),
+                   (launch_gemm<T>(false, true, true, below, cols, b, T(-1), panel, lda, panel, lda, T(1),
+                                   panel + b, lda, info, s)));
+        }
       }
       const int m2 = n - k0 - kb;
       if (m2 > 0) {
         T* p = a + (int64_t)(k0 + kb) * lda + k0;
-        LLH_TRY(launch_gemm<T>(false, true, true, m2, m2, kb, T(-1), p, lda, p, lda, T(1), p + kb, lda, info,
-                               s));
+        // Algorithmic work of the lower-triangle SYRK: m2(m2+1)/2 outputs, each a
+        // kb-term dot product (2 flops per term); bytes: read and write the lower
+        // triangle of C once, read the m2 x kb panel once.
+        const double outs = (double)m2 * (m2 + 1) / 2.0;
+        LLH_PROF(kTrailingSyrk, 2.0 * outs * kb, es * (2.0 * outs + (double)m2 * kb),
+                 (launch_gemm<T>(false, true, true, m2, m2, kb, T(-1), p, lda, p, lda, T(1), p + kb, lda,
+                                 info, s)));
       }
     }
   }
@@ -59,41 +81,46 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
 // done a block of rows at a time).
 template <typename T>
 hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t lda, int m, T* b, int64_t ldb,
-                        hipStream_t s) {
+                        hipStream_t s, Profiler* prof) {
   if (n <= 0 || m <= 0) return hipSuccess;
+  const double es = (double)sizeof(T);
   const int nblk = (n + kNB - 1) / kNB;
   if (!upper_op) {  // forward
     for (int bi = 0; bi < nblk; ++bi) {
       const int i0 = bi * kNB, bs = std::min(kNB, n - i0);
       const T* tkk = a + (int64_t)i0 * lda + i0;
-      LLH_TRY(launch_trsv_block<T>(false, trans, bs, tkk, lda, m, b + (int64_t)i0 * ldb, ldb, s));
+      LLH_PROF(kSolveDiag, (double)bs * bs * m, es * ((double)bs * bs / 2.0 + 2.0 * bs * m),
+               (launch_trsv_block<T>(false, trans, bs, tkk, lda, m, b + (int64_t)i0 * ldb, ldb, s)));
       const int rest = n - i0 - bs;
       if (rest <= 0) continue;
       // B[i0+bs:, :] -= op(A)[i0+bs:, i0:i0+bs] * X[i0:i0+bs, :]
       const T* blk = trans ? a + (int64_t)i0 * lda + (i0 + bs) : a + (int64_t)(i0 + bs) * lda + i0;
-      LLH_TRY(launch_gemm<T>(trans, false, false, rest, m, bs, T(-1), blk, lda, b + (int64_t)i0 * ldb, ldb,
-                             T(1), b + (int64_t)(i0 + bs) * ldb, ldb, nullptr, s));
+      LLH_PROF(kSolveGemm, 2.0 * rest * m * bs, es * ((double)rest * bs + 2.0 * rest * m + (double)bs * m),
+               (launch_gemm<T>(trans, false, false, rest, m, bs, T(-1), blk, lda, b + (int64_t)i0 * ldb, ldb,
+                               T(1), b + (int64_t)(i0 + bs) * ldb, ldb, nullptr, s)));
     }
   } else {  // backward
     for (int bi = nblk - 1; bi >= 0; --bi) {
       const int i0 = bi * kNB, bs = std::min(kNB, n - i0);
       const T* tkk = a + (int64_t)i0 * lda + i0;
-      LLH_TRY(launch_trsv_block<T>(true, trans, bs, tkk, lda, m, b + (int64_t)i0 * ldb, ldb, s));
+      LLH_PROF(kSolveDiag, (double)bs * bs * m, es * ((double)bs * bs / 2.0 + 2.0 * bs * m),
+               (launch_trsv_block<T>(true, trans, bs, tkk, lda, m, b + (int64_t)i0 * ldb, ldb, s)));
       if (i0 == 0) continue;
       // B[:i0, :] -= op(A)[:i0, i0:i0+bs] * X[i0:i0+bs, :]
       const T* blk = trans ? a + (int64_t)i0 * lda : a + i0;
-      LLH_TRY(launch_gemm<T>(trans, false, false, i0, m, bs, T(-1), blk, lda, b + (int64_t)i0 * ldb, ldb,
-                             T(1), b, ldb, nullptr, s));
+      LLH_PROF(kSolveGemm, 2.0 * i0 * m * bs, es * ((double)i0 * bs + 2.0 * i0 * m + (double)bs * m),
+               (launch_gemm<T>(trans, false, false, i0, m, bs, T(-1), blk, lda, b + (int64_t)i0 * ldb, ldb,
+                               T(1), b, ldb, nullptr, s)));
     }
   }
   return hipSuccess;
 }
 
-template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool, hipStream_t);
-template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, hipStream_t);
+template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool, hipStream_t, Profiler*);
+template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, hipStream_t, Profiler*);
 template hipError_t trsm_enqueue<double>(bool, bool, int, const double*, int64_t, int, double*, int64_t,
-                                         hipStream_t);
+                                         hipStream_t, Profiler*);
 template hipError_t trsm_enqueue<float>(bool, bool, int, const float*, int64_t, int, float*, int64_t,
-                                        hipStream_t);
+                                        hipStream_t, Profiler*);
 
 }  // namespace llh

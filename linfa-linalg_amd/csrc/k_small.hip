@@ -25,27 +25,30 @@ namespace {
 // Packed column-major lower triangle: entry (j, k), j >= k.
 __device__ __forceinline__ int pk(int n, int j, int k) { return k * n - ((k * (k - 1)) >> 1) + (j - k); }
 
-// Factors the n x n matrix at `a` (row-major, lda) in the LDS buffer W.
+// Factors the n x n matrix at `a` (row-major, lda) in LDS: Ap receives the
+// lower triangle of A, W the pending sums and then L (both packed column-major).
 // Returns -1 on success or the first row j whose pivot fails `d <= 0`.  On
 // return W holds every final column (all columns on success, columns < fail
 // otherwise) and Dg the square roots of the passed pivots.
 template <typename T, int NT>
-__device__ int factor_exact(const T* __restrict__ a, int64_t lda, int n, T* W, T* Dg) {
+__device__ int factor_exact(const T* __restrict__ a, int64_t lda, int n, T* W, T* Ap, T* Dg) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   constexpr int NW = NT / 64;
   const int tot = (n * (n + 1)) >> 1;
   for (int e = tid; e < tot; e += NT) W[e] = T(0);
+  for (int j = wave; j < n; j += NW)
+    for (int k = lane; k <= j; k += 64) Ap[pk(n, j, k)] = a[(int64_t)j * lda + k];
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     // Every thread evaluates the pivot identically, so no broadcast is needed.
-    const T d = a[(int64_t)k * lda + k] - W[pk(n, k, k)];
+    const T d = Ap[pk(n, k, k)] - W[pk(n, k, k)];
     if (d <= T(0)) return k;  // cholesky.rs:69; NaN passes
     const T piv = sqrt_rn(d);
     if (tid == 0) Dg[k] = piv;
     for (int j = k + 1 + tid; j < n; j += NT) {
       const int o = pk(n, j, k);
-      W[o] = (a[(int64_t)j * lda + k] - W[o]) / piv;  // cholesky.rs:62
+      W[o] = (Ap[o] - W[o]) / piv;  // cholesky.rs:62
     }
     __syncthreads();
     // Pending sums: S[j,k'] += L[k',k] * L[j,k] for k < k' <= j (k' == j is d_j).
@@ -81,8 +84,9 @@ __global__ __launch_bounds__(NT) void potrf_exact_single(T* a, int64_t lda, int 
                                                          int64_t info_base, const int32_t* gate) {
   if (gate != nullptr && *gate != kNoFail) return;
   __shared__ T W[(NMAX * (NMAX + 1)) / 2];
+  __shared__ T Ap[(NMAX * (NMAX + 1)) / 2];
   __shared__ T Dg[NMAX];
-  const int fail = factor_exact<T, NT>(a, lda, n, W, Dg);
+  const int fail = factor_exact<T, NT>(a, lda, n, W, Ap, Dg);
   __syncthreads();
   write_back<T, NT>(a, lda, n, fail, W, Dg);
   if (fail >= 0 && threadIdx.x == 0) atomicMin(info, (int32_t)(info_base + fail));
@@ -98,10 +102,11 @@ __global__ __launch_bounds__(NT) void potrf_potrs_exact_batched(const SmallDesc<
   const int n = ds.n;
   if (n > NMAX) return;  // the host never routes a larger matrix here
   __shared__ T W[(NMAX * (NMAX + 1)) / 2];
+  __shared__ T Ap[(NMAX * (NMAX + 1)) / 2];
   __shared__ T Dg[NMAX];
   __shared__ T Bs[NMAX * MCH];
   __shared__ T Xs[NMAX * MCH];
-  const int fail = factor_exact<T, NT>(ds.a, ds.lda, n, W, Dg);
+  const int fail = factor_exact<T, NT>(ds.a, ds.lda, n, W, Ap, Dg);
   __syncthreads();
   write_back<T, NT>(ds.a, ds.lda, n, fail, W, Dg);
   if (threadIdx.x == 0) ds.info[0] = fail < 0 ? kNoFail : fail;

@@ -3,12 +3,15 @@
 // trsm_rows: the panel step of the blocked POTRF, X := X * L11^-T, i.e. for
 // every row x of the panel, x[c] = (a[c] - sum_{p<c} x[p] * L11[c,p]) / L11[c,c]
 // (the division of cholesky.rs:62 applied to a whole block of rows).  One
-// thread owns one row, held in registers; L11 sits in LDS and every read of it
-// is a wave-wide broadcast.
+// thread owns one row, held in registers; L11 sits in LDS transposed and every
+// read of it is a wave-wide broadcast.  The sums are accumulated column-sweep
+// style (after x[c] is final, every later x[q] -= x[c] * L11[q,c]), so each
+// step issues up to 63 independent FMAs instead of one dependent chain.
 //
-// trsv_block: the diagonal-block step of the blocked triangular solve
-// (triangular.rs:113-125 restricted to b <= 64 rows): one thread owns one
-// right-hand-side column, held in registers.
+// trsv_block: the diagonal-block step of the blocked triangular solve: one
+// thread owns one right-hand-side column, held in registers, swept exactly as
+// triangular.rs:113-125 sweeps it (divide by the pivot, then update the rest of
+// the column), restricted to b <= 64 rows.
 #include "llh_common.h"
 #include "llh_kernels.h"
 
@@ -20,11 +23,11 @@ __global__ __launch_bounds__(64) void trsm_rows_kernel(int m, int b, T* __restri
                                                        const T* __restrict__ L, int64_t ldl,
                                                        const int32_t* __restrict__ gate) {
   if (gate != nullptr && *gate != kNoFail) return;
-  __shared__ T Ls[kNB * kNB];
+  __shared__ __attribute__((aligned(16))) T Lt[kNB * kNB];  // Lt[c * kNB + q] = L11[q, c]
   const int tid = threadIdx.x;
   for (int e = tid; e < kNB * kNB; e += 64) {
-    const int r = e / kNB, c = e % kNB;
-    Ls[e] = (r < b && c <= r) ? L[(int64_t)r * ldl + c] : T(0);
+    const int q = e / kNB, c = e % kNB;  // coalesced read of row q
+    Lt[c * kNB + q] = (q < b && c <= q) ? L[(int64_t)q * ldl + c] : T(0);
   }
   __syncthreads();
   const int row = blockIdx.x * 64 + tid;
@@ -50,10 +53,9 @@ __global__ __launch_bounds__(64) void trsm_rows_kernel(int m, int b, T* __restri
 #pragma unroll
   for (int c = 0; c < kNB; ++c) {
     if (c < b) {
-      T s = x[c];
+      x[c] = x[c] / Lt[c * kNB + c];
 #pragma unroll
-      for (int p = 0; p < c; ++p) s -= x[p] * Ls[c * kNB + p];
-      x[c] = s / Ls[c * kNB + c];
+      for (int q = c + 1; q < kNB; ++q) x[q] -= x[c] * Lt[c * kNB + q];
     }
   }
 #pragma unroll
@@ -64,12 +66,15 @@ __global__ __launch_bounds__(64) void trsm_rows_kernel(int m, int b, T* __restri
 template <typename T, bool UPPER_OP>
 __global__ __launch_bounds__(64) void trsv_block_kernel(int b, bool trans, const T* __restrict__ Tm,
                                                         int64_t ldt, int m, T* __restrict__ Y, int64_t ldy) {
-  __shared__ T Ts[kNB * kNB];  // op(T), only its triangle
+  // St[c * kNB + r] = op(T)[r, c], only op(T)'s triangle: column c of op(T) is
+  // contiguous, which is what the column sweep reads.
+  __shared__ __attribute__((aligned(16))) T St[kNB * kNB];
   const int tid = threadIdx.x;
   for (int e = tid; e < kNB * kNB; e += 64) {
-    const int r = e / kNB, c = e % kNB;
+    const int i = e / kNB, j = e % kNB;               // reads T[i, j], coalesced over j
+    const int r = trans ? j : i, c = trans ? i : j;   // T[i, j] is op(T)[r, c]
     const bool in = r < b && c < b && (UPPER_OP ? c >= r : c <= r);
-    Ts[e] = in ? (trans ? Tm[(int64_t)c * ldt + r] : Tm[(int64_t)r * ldt + c]) : T(0);
+    St[c * kNB + r] = in ? Tm[(int64_t)i * ldt + j] : T(0);
   }
   __syncthreads();
   const int col = blockIdx.x * 64 + tid;
@@ -81,20 +86,18 @@ __global__ __launch_bounds__(64) void trsv_block_kernel(int b, bool trans, const
 #pragma unroll
     for (int r = 0; r < kNB; ++r) {
       if (r < b) {
-        T s = y[r];
+        y[r] = y[r] / St[r * kNB + r];
 #pragma unroll
-        for (int p = 0; p < r; ++p) s -= Ts[r * kNB + p] * y[p];
-        y[r] = s / Ts[r * kNB + r];
+        for (int p = r + 1; p < kNB; ++p) y[p] -= y[r] * St[r * kNB + p];
       }
     }
   } else {
 #pragma unroll
     for (int r = kNB - 1; r >= 0; --r) {
       if (r < b) {
-        T s = y[r];
+        y[r] = y[r] / St[r * kNB + r];
 #pragma unroll
-        for (int p = r + 1; p < kNB; ++p) s -= Ts[r * kNB + p] * y[p];
-        y[r] = s / Ts[r * kNB + r];
+        for (int p = 0; p < r; ++p) y[p] -= y[r] * St[r * kNB + p];
       }
     }
   }

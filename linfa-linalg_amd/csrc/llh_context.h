@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "llh.h"
+#include "llh_profile.h"
 
 struct llh_context {
   static constexpr uint32_t kMagic = 0x4c4c4843;  // "LLHC"
@@ -20,6 +21,8 @@ struct llh_context {
     size_t bytes = 0;
   };
   Buffer scratch[4];  // grow-only device buffers of the plain forms
+  llh::Profiler prof;  // live kernel timing (off unless enabled)
+  llh::Profiler* profiler() { return prof.on ? &prof : nullptr; }
 };
 
 namespace llh {

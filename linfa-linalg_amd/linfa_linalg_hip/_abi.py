@@ -27,6 +27,15 @@ _sz, _pd, _vp, _int = ctypes.c_size_t, ctypes.c_ssize_t, ctypes.c_void_p, ctypes
 _err = ctypes.POINTER(LlhError)
 
 
+class KernelStats(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double),
+                ("flops", ctypes.c_double), ("bytes", ctypes.c_double)]
+
+
+KERNEL_CLASSES = {"trailing_syrk": 0, "panel_update": 1, "diag_factor": 2, "panel_trsm": 3,
+                  "solve_gemm": 4, "solve_diag": 5}
+
+
 def _declare(lib):
     lib.llh_context_create.argtypes = [_int, ctypes.POINTER(ctypes.c_void_p), _err]
     lib.llh_context_destroy.argtypes = [_vp]
@@ -36,6 +45,8 @@ def _declare(lib):
     lib.llh_context_stream.restype = _vp
     lib.llh_status_string.argtypes = [ctypes.c_int32]
     lib.llh_status_string.restype = ctypes.c_char_p
+    lib.llh_context_set_profiling.argtypes = [_vp, _int, _err]
+    lib.llh_context_kernel_stats.argtypes = [_vp, _int, ctypes.POINTER(KernelStats), _err]
     for s in ("d", "s"):
         f = lambda n: getattr(lib, f"llh_{n}_{s}")  # noqa: E731
         f("cholesky_inplace_dirty").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _err]

@@ -73,3 +73,21 @@ def solvec(a, b):
     _call("solvec", a, n, ctypes.c_void_p(a.data_ptr()), a.stride(0), m,
           ctypes.c_void_p(b.data_ptr()), b.stride(0))
     return b
+
+
+def set_profiling(enable):
+    c = context()
+    err = _abi.LlhError()
+    _raise(c._lib.llh_context_set_profiling(c.handle, int(bool(enable)), ctypes.byref(err)), err)
+
+
+def kernel_stats():
+    """{class name: (launches, total_ms, flops, bytes)} of the profiled launches."""
+    c = context()
+    out = {}
+    for name, cls in _abi.KERNEL_CLASSES.items():
+        st = _abi.KernelStats()
+        err = _abi.LlhError()
+        _raise(c._lib.llh_context_kernel_stats(c.handle, cls, ctypes.byref(st), ctypes.byref(err)), err)
+        out[name] = (st.launches, st.total_ms, st.flops, st.bytes)
+    return out

@@ -71,7 +71,7 @@ def test_seeded_fixtures_bit_identical():
             assert np.array_equal(x, fx[f"{tag}{n}_x"]), (tag, n)
 
 
-@pytest.mark.parametrize("n", [129, 200, 256, 300, 512, 777, 1024])
+@pytest.mark.parametrize("n", [129, 200, 256, 300, 512, 777, 1024, 1111, 1537])
 def test_blocked_f64_against_oracle(n):
     a = spd(n, n)
     l_ref = orc.cholesky(a)
@@ -99,7 +99,7 @@ def test_c1_config_n512():
     assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-13
 
 
-@pytest.mark.parametrize("n", [200, 700])
+@pytest.mark.parametrize("n", [200, 700, 1111])
 def test_blocked_f32(n):
     a = spd(n, n, np.float32)
     l = llh.cholesky(a)
