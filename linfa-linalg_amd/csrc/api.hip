@@ -74,9 +74,68 @@ hipError_t ctx_buffer(llh_context* ctx, int slot, size_t bytes, void** out) {
   return hipSuccess;
 }
 
+static hipError_t grow(llh_context* ctx, llh_context::Buffer& b, size_t bytes, bool* fresh) {
+  *fresh = false;
+  if (b.bytes >= bytes) return hipSuccess;
+  if (b.ptr != nullptr) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return e;
+    (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  const size_t want = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.ptr, want);
+  if (e != hipSuccess) return e;
+  b.bytes = want;
+  *fresh = true;
+  return hipSuccess;
+}
+
+hipError_t ctx_dinv(llh_context* ctx, int n, size_t es, void** out) {
+  bool fresh = false;
+  hipError_t e = grow(ctx, ctx->dinv, (size_t)dinv_elems(n) * es, &fresh);
+  *out = ctx->dinv.ptr;
+  return e;
+}
+
+hipError_t ctx_sync(llh_context* ctx, int n, int m, SolveSync* sync) {
+  const int64_t nb = (n + kNB - 1) / kNB, nq = (m + kSolveChunkHost - 1) / kSolveChunkHost;
+  const int64_t nflags = nb * std::max<int64_t>(nq, 1);
+  bool fresh = false;
+  hipError_t e = grow(ctx, ctx->sync, (size_t)(2 + nflags) * sizeof(int32_t), &fresh);
+  if (e != hipSuccess) return e;
+  if (fresh) {  // flags start at 0, below every epoch the context hands out
+    e = launch_fill_i32(static_cast<int32_t*>(ctx->sync.ptr), 0, (int64_t)(ctx->sync.bytes / sizeof(int32_t)),
+                        ctx->stream);
+    if (e != hipSuccess) return e;
+    ctx->epoch = 1;
+  }
+  sync->words = static_cast<int32_t*>(ctx->sync.ptr);
+  sync->nflags = (int64_t)(ctx->sync.bytes / sizeof(int32_t)) - 2;
+  sync->epoch = ctx->epoch;
+  return hipSuccess;
+}
+
+hipError_t ctx_read_solve_error(llh_context* ctx) {
+  if (ctx->sync.ptr == nullptr) {
+    ctx->h_info[2] = 0;
+    return hipSuccess;
+  }
+  return hipMemcpyAsync(ctx->h_info + 2, static_cast<int32_t*>(ctx->sync.ptr) + 1, sizeof(int32_t),
+                        hipMemcpyDeviceToHost, ctx->stream);
+}
+
 }  // namespace llh
 
 namespace {
+
+llh_status solve_stalled(llh_error* err) {
+  return set_error(err, LLH_ERR_HIP, 0, 0, -1, "triangular solve: a block wait exceeded its bound", 0);
+}
+
+// Runs after every persistent solve: records the advanced epoch.
+void commit_sync(llh_context* ctx, const SolveSync& s) { ctx->epoch = s.epoch; }
 
 #define CHECK_HIP(expr, where)                           \
   do {                                                   \
@@ -124,8 +183,11 @@ llh_status factor_plain(llh_context* ctx, size_t n, T* a, ptrdiff_t rs, ptrdiff_
   void* dv = nullptr;
   CHECK_HIP(ctx_buffer(ctx, 0, n * n * sizeof(T), &dv), "device buffer");
   T* d = static_cast<T*>(dv);
+  void* dinv = nullptr;
+  CHECK_HIP(ctx_dinv(ctx, (int)n, sizeof(T), &dinv), "inverse buffer");
   CHECK_HIP(hipMemcpyAsync(d, h.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
-  CHECK_HIP(potrf_enqueue<T>(d, (int64_t)n, (int)n, ctx->d_info, clean, ctx->stream), "potrf");
+  CHECK_HIP(potrf_enqueue<T>(d, (int64_t)n, (int)n, ctx->d_info, clean, static_cast<T*>(dinv), ctx->stream),
+            "potrf");
   CHECK_HIP(hipMemcpyAsync(h.data(), d, n * n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
             "status");
@@ -240,12 +302,20 @@ llh_status solve_triangular_plain(llh_context* ctx, size_t rows, size_t cols, co
   CHECK_HIP(hipMemcpyAsync(da, ha.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
   CHECK_HIP(hipMemcpyAsync(db, hb.data(), n * b_cols * sizeof(T), hipMemcpyHostToDevice, ctx->stream),
             "upload");
+  void* dinv = nullptr;
+  CHECK_HIP(ctx_dinv(ctx, (int)n, sizeof(T), &dinv), "inverse buffer");
+  SolveSync sync;
+  CHECK_HIP(ctx_sync(ctx, (int)n, (int)b_cols, &sync), "solve sync");
   CHECK_HIP(trsm_enqueue<T>(uplo == LLH_UPPER, false, (int)n, da, (int64_t)n, (int)b_cols, db,
-                            (int64_t)b_cols, ctx->stream),
+                            (int64_t)b_cols, nullptr, false, static_cast<T*>(dinv), &sync, ctx->stream,
+                            ctx->profiler()),
             "trsm");
+  commit_sync(ctx, sync);
+  CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   CHECK_HIP(hipMemcpyAsync(hb.data(), db, n * b_cols * sizeof(T), hipMemcpyDeviceToHost, ctx->stream),
             "download");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  if (ctx->h_info[2] != 0) return solve_stalled(err);
   scatter(hb.data(), n, b_cols, b, brs, bcs, [b_cols](size_t) { return b_cols; });
   return ok(err);
 }
@@ -262,12 +332,22 @@ llh_status solve_on_device(llh_context* ctx, size_t n, T* da, size_t m, T* db, l
     CHECK_HIP(launch_potrf_potrs_exact_batched<T>(static_cast<SmallDesc<T>*>(ddv), 1, (int)n, 1, ctx->stream),
               "small factor+solve");
   } else {
-    CHECK_HIP(potrf_enqueue<T>(da, (int64_t)n, (int)n, ctx->d_info, false, ctx->stream), "potrf");
-    CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, ctx->stream),
+    void* dv = nullptr;
+    CHECK_HIP(ctx_dinv(ctx, (int)n, sizeof(T), &dv), "inverse buffer");
+    T* dinv = static_cast<T*>(dv);
+    SolveSync sync;
+    CHECK_HIP(ctx_sync(ctx, (int)n, (int)m, &sync), "solve sync");
+    CHECK_HIP(potrf_enqueue<T>(da, (int64_t)n, (int)n, ctx->d_info, false, dinv, ctx->stream, ctx->profiler()),
+              "potrf");
+    CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, dinv, false, nullptr,
+                              &sync, ctx->stream, ctx->profiler()),
               "forward solve");
-    CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, ctx->stream),
+    CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, dinv, true, nullptr,
+                              &sync, ctx->stream, ctx->profiler()),
               "back solve");
+    commit_sync(ctx, sync);
   }
+  CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   return LLH_OK;
 }
 
@@ -325,6 +405,7 @@ llh_status solvec_plain(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdif
     scatter(ha.data(), fr + 1, n, a, rs, cs, [fr](size_t i) { return i < fr ? i + 1 : fr; });
     return not_pd(err, f);
   }
+  if (ctx->h_info[2] != 0) return solve_stalled(err);
   scatter(ha.data(), n, n, a, rs, cs, [](size_t i) { return i + 1; });  // dirty factor
   if (invert)
     scatter(hb.data(), n, n, inv_out, ors, ocs, [n](size_t) { return n; });
@@ -341,8 +422,10 @@ llh_status dev_potrf(llh_context* ctx, size_t n, T* a, size_t lda, int zero_uppe
   if (n == 0) return ok(err);
   if (a == nullptr || lda < n) return invalid(err, "null matrix or lda < n");
   if (n > 0x7fffffff) return invalid(err, "matrix too large");
-  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, zero_upper != 0, ctx->stream,
-                            ctx->profiler()),
+  void* dinv = nullptr;
+  CHECK_HIP(ctx_dinv(ctx, (int)n, sizeof(T), &dinv), "inverse buffer");
+  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, zero_upper != 0, static_cast<T*>(dinv),
+                            ctx->stream, ctx->profiler()),
             "potrf");
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
             "status");
@@ -357,10 +440,17 @@ llh_status dev_solve_triangular(llh_context* ctx, llh_uplo uplo, int trans, size
   if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
   if (n == 0 || m == 0) return ok(err);
   if (a == nullptr || b == nullptr || lda < n || ldb < m) return invalid(err, "bad matrix arguments");
+  void* dinv = nullptr;
+  CHECK_HIP(ctx_dinv(ctx, (int)n, sizeof(T), &dinv), "inverse buffer");
+  SolveSync sync;
+  CHECK_HIP(ctx_sync(ctx, (int)n, (int)m, &sync), "solve sync");
   CHECK_HIP(trsm_enqueue<T>(uplo == LLH_UPPER, trans != 0, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb,
-                            ctx->stream, ctx->profiler()),
+                            nullptr, false, static_cast<T*>(dinv), &sync, ctx->stream, ctx->profiler()),
             "trsm");
+  commit_sync(ctx, sync);
+  CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  if (ctx->h_info[2] != 0) return solve_stalled(err);
   return ok(err);
 }
 
@@ -370,22 +460,30 @@ llh_status dev_solvec(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T*
   if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
   if (n == 0) return ok(err);
   if (a == nullptr || lda < n || (m > 0 && (b == nullptr || ldb < m))) return invalid(err, "bad arguments");
-  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, false, ctx->stream, ctx->profiler()),
+  void* dv = nullptr;
+  CHECK_HIP(ctx_dinv(ctx, (int)n, sizeof(T), &dv), "inverse buffer");
+  T* dinv = static_cast<T*>(dv);
+  SolveSync sync;
+  CHECK_HIP(ctx_sync(ctx, (int)n, (int)std::max<size_t>(m, 1), &sync), "solve sync");
+  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, false, dinv, ctx->stream, ctx->profiler()),
             "potrf");
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
             "status");
   if (m > 0) {
     // Solves run regardless of the status and are discarded on failure; this
     // keeps one synchronisation per call.
-    CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, ctx->stream,
-                              ctx->profiler()),
+    CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, false,
+                              nullptr, &sync, ctx->stream, ctx->profiler()),
               "forward solve");
-    CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, ctx->stream,
-                              ctx->profiler()),
+    CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, true, nullptr,
+                              &sync, ctx->stream, ctx->profiler()),
               "back solve");
+    commit_sync(ctx, sync);
   }
+  CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
   if (ctx->h_info[0] != kNoFail) return not_pd(err, ctx->h_info[0]);
+  if (ctx->h_info[2] != 0) return solve_stalled(err);
   return ok(err);
 }
 
@@ -433,6 +531,8 @@ void llh_context_destroy(llh_context* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& b : ctx->scratch)
     if (b.ptr) (void)hipFree(b.ptr);
+  if (ctx->dinv.ptr) (void)hipFree(ctx->dinv.ptr);
+  if (ctx->sync.ptr) (void)hipFree(ctx->sync.ptr);
   if (ctx->d_info) (void)hipFree(ctx->d_info);
   if (ctx->h_info) (void)hipHostFree(ctx->h_info);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);

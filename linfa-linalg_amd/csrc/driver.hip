@@ -28,13 +28,15 @@ namespace llh {
 // matrix.  Matrices no larger than the LDS kernel's limit are factored by that
 // kernel alone, in the reference's exact operation order.
 template <typename T>
-hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, hipStream_t s,
+hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, T* dinv, hipStream_t s,
                          Profiler* prof) {
   const double es = (double)sizeof(T);
   LLH_TRY(launch_fill_i32(info, kNoFail, 1, s));
   if (n <= 0) return hipSuccess;
-  if (n <= small_max<T>()) {
+  if (n <= small_max<T>() || dinv == nullptr) {
+    if (n > small_max<T>()) return hipErrorInvalidValue;  // the blocked path needs the inverse buffer
     LLH_TRY(launch_potrf_exact_single<T>(a, lda, n, info, 0, nullptr, s));
+    if (dinv != nullptr) LLH_TRY(launch_trtri_blocks<T>(n, a, lda, false, false, dinv, info, s));
   } else {
     for (int k0 = 0; k0 < n; k0 += kNBO) {
       const int kb = std::min(kNBO, n - k0);
@@ -44,11 +46,18 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
         const double db = b;
         LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * (db + 1.0),
                  (launch_potrf_exact_single<T>(akk, lda, b, info, k, info, s)));
+        // inv(L11): serves this panel's solve and, later, the triangular solves.
+        T* dk = dinv + (int64_t)(k / kNB) * kNB * kNB;
+        LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * db,
+                 (launch_trtri_blocks<T>(b, akk, lda, false, false, dk, info, s)));
         const int below = n - k - b;
         if (below <= 0) continue;
         T* panel = akk + (int64_t)b * lda;  // rows k+b.., columns k..k+b
-        LLH_PROF(kPanelTrsm, (double)below * db * db, es * ((double)below * db * 2.0 + db * db / 2.0),
-                 (launch_trsm_rows<T>(below, b, panel, lda, akk, lda, info, s)));
+        // L21 = A21 * L11^-T as one MFMA product with the inverse, in place (each
+        // workgroup reads all of its rows' k before writing them).
+        LLH_PROF(kPanelTrsm, (double)below * db * db * 2.0, es * ((double)below * db * 2.0 + db * db),
+                 (launch_gemm<T>(false, true, false, below, b, b, T(1), panel, lda, dk, kNB, T(0), panel, lda,
+                                 info, s)));
         const int cols = k0 + kb - (k + b);
         if (cols > 0) {
           // rows below x cols, lower part of the top cols x cols square only
@@ -81,9 +90,27 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
 // done a block of rows at a time).
 template <typename T>
 hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t lda, int m, T* b, int64_t ldb,
-                        hipStream_t s, Profiler* prof) {
+                        const T* dinv, bool dinv_t, T* dinv_work, SolveSync* sync, hipStream_t s,
+                        Profiler* prof) {
   if (n <= 0 || m <= 0) return hipSuccess;
   const double es = (double)sizeof(T);
+  const int nblk0 = (n + kNB - 1) / kNB;
+  const int nq = (m + kSolveChunkHost - 1) / kSolveChunkHost;
+  if (sync != nullptr && m <= 64 && sync->nflags >= (int64_t)nblk0 * nq && (dinv != nullptr || dinv_work)) {
+    if (dinv == nullptr) {
+      LLH_PROF(kSolveDiag, (double)n * kNB * kNB / 3.0, es * (double)n * kNB,
+               (launch_trtri_blocks<T>(n, a, lda, trans, upper_op, dinv_work, nullptr, s)));
+      dinv = dinv_work;
+      dinv_t = false;
+    }
+    LLH_TRY(launch_fill_i32(sync->words, 0, 2, s));
+    const double fl = (double)n * n * m;
+    LLH_PROF(kSolveGemm, fl, es * ((double)n * n / 2.0 + 2.0 * n * m),
+             (launch_trsm_persistent<T>(n, m, a, lda, trans, !upper_op, dinv, dinv_t, b, ldb, sync->words,
+                                        sync->words + 2, sync->epoch, sync->words + 1, kSpinLimit, s)));
+    sync->epoch += 1;
+    return hipSuccess;
+  }
   const int nblk = (n + kNB - 1) / kNB;
   if (!upper_op) {  // forward
     for (int bi = 0; bi < nblk; ++bi) {
@@ -116,11 +143,12 @@ hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t ld
   return hipSuccess;
 }
 
-template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool, hipStream_t, Profiler*);
-template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, hipStream_t, Profiler*);
+template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool, double*, hipStream_t,
+                                          Profiler*);
+template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, float*, hipStream_t, Profiler*);
 template hipError_t trsm_enqueue<double>(bool, bool, int, const double*, int64_t, int, double*, int64_t,
-                                         hipStream_t, Profiler*);
+                                         const double*, bool, double*, SolveSync*, hipStream_t, Profiler*);
 template hipError_t trsm_enqueue<float>(bool, bool, int, const float*, int64_t, int, float*, int64_t,
-                                        hipStream_t, Profiler*);
+                                        const float*, bool, float*, SolveSync*, hipStream_t, Profiler*);
 
 }  // namespace llh

@@ -98,11 +98,11 @@ struct SlabLoader {
 };
 
 template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, bool TRI>
-__global__ __launch_bounds__(WM* WN * 64) void gemm_kernel(int M, int N, int K, T alpha,
-                                                           const T* __restrict__ A, int64_t lda,
-                                                           const T* __restrict__ B, int64_t ldb, T beta,
-                                                           T* __restrict__ C, int64_t ldc,
-                                                           const int32_t* __restrict__ gate) {
+// A and C may alias (the in-place panel solve X := X * Linv^T): every read of
+// A precedes the epilogue's writes, and a workgroup only touches its own rows.
+__global__ __launch_bounds__(WM* WN * 64) void gemm_kernel(int M, int N, int K, T alpha, const T* A, int64_t lda,
+                                                           const T* __restrict__ B, int64_t ldb, T beta, T* C,
+                                                           int64_t ldc, const int32_t* __restrict__ gate) {
   constexpr int NTHR = WM * WN * 64;
   constexpr int TM = BM / WM, TN = BN / WN;
   constexpr int FM = TM / 16, FN = TN / 16;

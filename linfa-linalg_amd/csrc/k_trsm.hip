@@ -106,6 +106,199 @@ __global__ __launch_bounds__(64) void trsv_block_kernel(int b, bool trans, const
     if (r < b) Y[(int64_t)r * ldy + col] = y[r];
 }
 
+// ---------------------------------------------------------------------------
+// trtri64: inverses of the 64 x 64 diagonal blocks of op(A) (the last block may
+// be smaller; it is padded with an identity tail).  One workgroup per block.
+// The block is copied to LDS in lower-triangular form M (op(A)'s block, or its
+// transpose when that block is upper triangular); inv(M) is built by recursive
+// doubling: the four 16 x 16 diagonal blocks are inverted by substitution (one
+// thread per column), then pairs are combined with
+//   [P 0; C Q]^-1 = [P^-1 0; -Q^-1 C P^-1  Q^-1]
+// at 32 and at 64.  The result is written row-major, transposed back when op(A)'s
+// block is upper, with exact zeros outside the triangle.
+template <typename T>
+__global__ __launch_bounds__(256) void trtri64_kernel(int n, const T* __restrict__ A, int64_t lda, bool trans,
+                                                      bool upper, T* __restrict__ out,
+                                                      const int32_t* __restrict__ gate) {
+  if (gate != nullptr && *gate != kNoFail) return;
+  constexpr int S = kNB + 1;
+  __shared__ T Ms[kNB * S];
+  __shared__ T Xs[kNB * S];
+  __shared__ T Ts[kNB * S];
+  const int tid = threadIdx.x;
+  const int o = blockIdx.x * kNB;
+  const int b = min(kNB, n - o);
+  // Element (r, c) of M, r >= c: op(A)[o+r, o+c] (lower) or op(A)[o+c, o+r] (upper);
+  // with op(A) = A^T when trans, M[r][c] is A[o+c][o+r] exactly when upper != trans.
+  const bool flip = (upper != trans);
+  for (int e = tid; e < kNB * kNB; e += 256) {
+    const int i = e / kNB, j = e % kNB;  // coalesced over j in A's storage
+    const int r = flip ? j : i, c = flip ? i : j;
+    T v = T(0);
+    if (r < b && c < b) {
+      if (c <= r) v = A[(int64_t)(o + i) * lda + (o + j)];
+    } else if (r == c) {
+      v = T(1);
+    }
+    Ms[r * S + c] = v;
+    Xs[r * S + c] = T(0);
+  }
+  __syncthreads();
+  // 16 x 16 diagonal blocks: thread (blk, c) builds column c of inv(D_blk).
+  if (tid < kNB) {
+    const int base = (tid >> 4) * 16, c = tid & 15;
+    T x[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      T s = T(0);
+#pragma unroll
+      for (int p = 0; p < r; ++p) s += (p >= c) ? Ms[(base + r) * S + base + p] * x[p] : T(0);
+      const T d = Ms[(base + r) * S + base + r];
+      x[r] = (r < c) ? T(0) : (r == c ? T(1) / d : -s / d);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) Xs[(base + r) * S + base + c] = x[r];
+  }
+  __syncthreads();
+  // Combine at h = 16 (two pairs) and h = 32 (one pair).
+#pragma unroll
+  for (int h = 16; h <= 32; h *= 2) {
+    const int pairs = kNB / (2 * h);
+    // T = C * P^-1, C = M[base+h.., base..base+h), P^-1 = X[base.., base..]
+    for (int e = tid; e < pairs * h * h; e += 256) {
+      const int pr = e / (h * h), rem = e % (h * h), i = rem / h, j = rem % h;
+      const int base = pr * 2 * h;
+      T s = T(0);
+      for (int k = j; k < h; ++k) s += Ms[(base + h + i) * S + base + k] * Xs[(base + k) * S + base + j];
+      Ts[(base + h + i) * S + base + j] = s;
+    }
+    __syncthreads();
+    // X21 = -Q^-1 * T, Q^-1 = X[base+h.., base+h..]
+    for (int e = tid; e < pairs * h * h; e += 256) {
+      const int pr = e / (h * h), rem = e % (h * h), i = rem / h, j = rem % h;
+      const int base = pr * 2 * h;
+      T s = T(0);
+      for (int k = 0; k <= i; ++k) s += Xs[(base + h + i) * S + base + h + k] * Ts[(base + h + k) * S + base + j];
+      Xs[(base + h + i) * S + base + j] = -s;
+    }
+    __syncthreads();
+  }
+  T* dst = out + (int64_t)blockIdx.x * kNB * kNB;
+  for (int e = tid; e < kNB * kNB; e += 256) {
+    const int r = e / kNB, c = e % kNB;
+    dst[e] = upper ? Xs[c * S + r] : Xs[r * S + c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// trsm_persistent: op(A) X = B in place for up to kSolveChunk right-hand-side
+// columns per workgroup, one launch for the whole solve.
+//
+// Workgroups take tickets in launch order; ticket t is (position p, column chunk
+// q).  Position p is block row w = p (forward, op(A) lower) or w = nb-1-p
+// (backward).  The workgroup accumulates acc = B_w - sum_j op(A)_wj X_j over the
+// blocks j that precede w in solve order, waiting for each X_j on its flag, then
+// forms X_w = inv(op(A)_ww) acc with the precomputed inverse, stores it and
+// raises flag (w, q).  A workgroup only ever waits for positions taken before
+// its own, so the chain always advances whatever the residency.  Flags hold the
+// epoch of the call that raised them, so they never need clearing.  The wait is
+// bounded: past spin_limit polls (or once another workgroup has given up) the
+// workgroup records an error and returns without raising its flag.
+constexpr int kSolveChunk = kSolveChunkHost;
+
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_persistent_kernel(int n, int m, const T* __restrict__ A,
+                                                              int64_t lda, bool trans, bool lower_op,
+                                                              const T* __restrict__ Dinv, bool dinv_t,
+                                                              T* B, int64_t ldb, int32_t* ticket,
+                                                              int32_t* flags, int32_t epoch, int32_t* err,
+                                                              int64_t spin_limit) {
+  constexpr int S = kNB + 1, MC = kSolveChunk, SC = MC + 1;
+  __shared__ T As[kNB * S];
+  __shared__ T Ys[kNB * SC];
+  __shared__ int s_ticket, s_abort;
+  const int tid = threadIdx.x;
+  const int nb = (n + kNB - 1) / kNB, nq = (m + MC - 1) / MC;
+  if (tid == 0) {
+    s_ticket = atomicAdd(ticket, 1);
+    s_abort = 0;
+  }
+  __syncthreads();
+  const int tk = s_ticket;
+  if (tk >= nb * nq) return;
+  const int p = tk / nq, q = tk % nq;
+  const int w = lower_op ? p : nb - 1 - p;
+  const int w0 = w * kNB, bw = min(kNB, n - w0);
+  const int c0 = q * MC, mc = min(MC, m - c0);
+  // This thread's outputs: row r, columns cc .. cc+3 of the chunk.
+  const int r = tid >> 2, cc = (tid & 3) * 4;
+  T acc[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    acc[e] = (r < bw && cc + e < mc) ? B[(int64_t)(w0 + r) * ldb + c0 + cc + e] : T(0);
+
+  for (int s = 0; s < p; ++s) {
+    const int j = lower_op ? s : nb - 1 - s;
+    const int j0 = j * kNB, bj = min(kNB, n - j0);
+    // op(A)[w0+r, j0+k]: A[w0+r][j0+k], or A[j0+k][w0+r] when trans.
+    for (int e = tid; e < kNB * kNB; e += 256) {
+      const int i = e / kNB, jj = e % kNB;  // coalesced over jj
+      const int rr = trans ? jj : i, kk = trans ? i : jj;
+      T v = T(0);
+      if (rr < bw && kk < bj) v = trans ? A[(int64_t)(j0 + i) * lda + w0 + jj] : A[(int64_t)(w0 + i) * lda + j0 + jj];
+      As[rr * S + kk] = v;
+    }
+    if (tid == 0) {
+      int64_t it = 0;
+      while (__hip_atomic_load(&flags[j * nq + q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++it > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          atomicExch(err, 1);
+          s_abort = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __syncthreads();
+    if (s_abort) return;
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    for (int e = tid; e < kNB * MC; e += 256) {
+      const int k = e / MC, c = e % MC;
+      Ys[k * SC + c] = (k < bj && c < mc) ? __builtin_nontemporal_load(&B[(int64_t)(j0 + k) * ldb + c0 + c]) : T(0);
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int k = 0; k < kNB; ++k) {
+      const T a = As[r * S + k];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[e] -= a * Ys[k * SC + cc + e];
+    }
+    __syncthreads();
+  }
+  // X_w = inv(op(A)_ww) acc
+#pragma unroll
+  for (int e = 0; e < 4; ++e) Ys[r * SC + cc + e] = acc[e];
+  const T* D = Dinv + (int64_t)w * kNB * kNB;
+  for (int e = tid; e < kNB * kNB; e += 256) {
+    const int i = e / kNB, jj = e % kNB;
+    As[(dinv_t ? jj : i) * S + (dinv_t ? i : jj)] = D[e];
+  }
+  __syncthreads();
+  T x[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll 8
+  for (int k = 0; k < kNB; ++k) {
+    const T d = As[r * S + k];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) x[e] += d * Ys[k * SC + cc + e];
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    if (r < bw && cc + e < mc) B[(int64_t)(w0 + r) * ldb + c0 + cc + e] = x[e];
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(&flags[w * nq + q], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 }  // namespace
 
 template <typename T>
@@ -135,6 +328,39 @@ hipError_t launch_trsv_block(bool upper_op, bool trans, int b, const T* Tm, int6
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_trtri_blocks(int n, const T* A, int64_t lda, bool trans, bool upper, T* out,
+                               const int32_t* gate, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((trtri64_kernel<T>), dim3((n + kNB - 1) / kNB), dim3(256), 0, s, n, A, lda, trans, upper,
+                     out, gate);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_trsm_persistent(int n, int m, const T* A, int64_t lda, bool trans, bool lower_op,
+                                  const T* Dinv, bool dinv_t, T* B, int64_t ldb, int32_t* ticket,
+                                  int32_t* flags, int32_t epoch, int32_t* err, int64_t spin_limit,
+                                  hipStream_t s) {
+  if (n <= 0 || m <= 0) return hipSuccess;
+  const int nb = (n + kNB - 1) / kNB, nq = (m + kSolveChunk - 1) / kSolveChunk;
+  hipError_t e = launch_fill_i32(ticket, 0, 1, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((trsm_persistent_kernel<T>), dim3(nb * nq), dim3(256), 0, s, n, m, A, lda, trans, lower_op,
+                     Dinv, dinv_t, B, ldb, ticket, flags, epoch, err, spin_limit);
+  return hipGetLastError();
+}
+
+template hipError_t launch_trtri_blocks<double>(int, const double*, int64_t, bool, bool, double*, const int32_t*,
+                                                hipStream_t);
+template hipError_t launch_trtri_blocks<float>(int, const float*, int64_t, bool, bool, float*, const int32_t*,
+                                               hipStream_t);
+template hipError_t launch_trsm_persistent<double>(int, int, const double*, int64_t, bool, bool, const double*,
+                                                   bool, double*, int64_t, int32_t*, int32_t*, int32_t, int32_t*,
+                                                   int64_t, hipStream_t);
+template hipError_t launch_trsm_persistent<float>(int, int, const float*, int64_t, bool, bool, const float*, bool,
+                                                  float*, int64_t, int32_t*, int32_t*, int32_t, int32_t*, int64_t,
+                                                  hipStream_t);
 template hipError_t launch_trsm_rows<double>(int, int, double*, int64_t, const double*, int64_t,
                                              const int32_t*, hipStream_t);
 template hipError_t launch_trsm_rows<float>(int, int, float*, int64_t, const float*, int64_t, const int32_t*,

@@ -18,6 +18,11 @@ constexpr int32_t kNoFail = 0x7fffffff;
 constexpr int kNB = 64;
 constexpr int kNBO = 256;
 
+// Right-hand-side columns per workgroup of the one-launch triangular solve, and
+// the bound on its flag waits (2^22 polls: well over 0.1 s, far above any real wait).
+constexpr int kSolveChunkHost = 16;
+constexpr int64_t kSpinLimit = 1ll << 22;
+
 // Largest n the single-workgroup LDS kernel takes (the packed lower triangle of
 // an f64 matrix must fit in 160 KiB of LDS with room to spare).
 constexpr int kSmallMaxD = 128;

@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "llh.h"
+#include "llh_driver.h"
 #include "llh_profile.h"
 
 struct llh_context {
@@ -21,6 +22,9 @@ struct llh_context {
     size_t bytes = 0;
   };
   Buffer scratch[4];  // grow-only device buffers of the plain forms
+  Buffer dinv;        // inverses of the 64 x 64 diagonal blocks
+  Buffer sync;        // ticket / error / flags of the one-launch solve
+  int32_t epoch = 1;
   llh::Profiler prof;  // live kernel timing (off unless enabled)
   llh::Profiler* profiler() { return prof.on ? &prof : nullptr; }
 };
@@ -35,4 +39,10 @@ llh_status not_square(llh_error* err, size_t rows, size_t cols);
 llh_status not_pd(llh_error* err, int64_t col);
 llh_status wrong_rows(llh_error* err, size_t expected, size_t actual);
 hipError_t ctx_buffer(llh_context* ctx, int slot, size_t bytes, void** out);
+// Grow-only diagonal-inverse buffer for order n (elements of type size es).
+hipError_t ctx_dinv(llh_context* ctx, int n, size_t es, void** out);
+// Solve-sync words sized for order n and m right-hand sides; fills *sync.
+hipError_t ctx_sync(llh_context* ctx, int n, int m, SolveSync* sync);
+// Enqueues the read-back of the solve error word into h_info[2].
+hipError_t ctx_read_solve_error(llh_context* ctx);
 }  // namespace llh

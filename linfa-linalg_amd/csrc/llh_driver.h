@@ -6,16 +6,35 @@
 
 namespace llh {
 
+// Device words of the one-launch triangular solve: ticket counter, error word
+// and one flag per (64-row block, 16-column chunk); epoch is a value the flags
+// have never held (the context increments it per launch).
+struct SolveSync {
+  int32_t* words = nullptr;  // [0] ticket, [1] error, [2..] flags
+  int64_t nflags = 0;
+  int32_t epoch = 0;
+};
+
+// Number of elements of the diagonal-block inverse buffer for order n.
+inline int64_t dinv_elems(int n) { return (int64_t)((n + kNB - 1) / kNB) * kNB * kNB; }
+
 // Enqueues the in-place lower Cholesky of a (row-major, lda) on stream s.
-// *info receives kNoFail or the first row whose pivot fails `d <= 0`.
+// *info receives kNoFail or the first row whose pivot fails `d <= 0`.  When
+// dinv is non-null it receives inv(L_ii) of every 64 x 64 diagonal block (the
+// blocked path produces them on the way; they serve the panel solves and the
+// later triangular solves).
 template <typename T>
-hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, hipStream_t s,
+hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, T* dinv, hipStream_t s,
                          Profiler* prof = nullptr);
 
 // Enqueues op(A) X = B in place (B row-major n x m); upper_op names the triangle
-// of op(A) and trans != 0 means op(A) = A^T.
+// of op(A) and trans != 0 means op(A) = A^T.  dinv, when non-null, holds the
+// inverses of op(A)'s 64 x 64 diagonal blocks (transposed when dinv_t);
+// otherwise they are computed into dinv_work.  With sync and m <= 64 the solve
+// is one persistent launch; otherwise it is a sequence of blocked launches.
 template <typename T>
 hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t lda, int m, T* b, int64_t ldb,
-                        hipStream_t s, Profiler* prof = nullptr);
+                        const T* dinv, bool dinv_t, T* dinv_work, SolveSync* sync, hipStream_t s,
+                        Profiler* prof = nullptr);
 
 }  // namespace llh

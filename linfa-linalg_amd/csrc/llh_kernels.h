@@ -49,6 +49,21 @@ template <typename T>
 hipError_t launch_trsv_block(bool upper_op, bool trans, int b, const T* Tm, int64_t ldt, int m, T* Y,
                              int64_t ldy, hipStream_t s);
 
+// k_trsm.hip: inverses of the 64 x 64 diagonal blocks of op(A) (n x n), written
+// as consecutive row-major 64 x 64 blocks at out; upper names op(A)'s triangle.
+template <typename T>
+hipError_t launch_trtri_blocks(int n, const T* A, int64_t lda, bool trans, bool upper, T* out,
+                               const int32_t* gate, hipStream_t s);
+// k_trsm.hip: one-launch blocked solve op(A) X = B (B row-major n x m) using the
+// diagonal-block inverses Dinv (transposed when dinv_t).  ticket/flags/err are
+// device words (flags: one per 64-row block and 16-column chunk), epoch a value
+// the flags have never held.  *err becomes non-zero if a wait exceeded spin_limit.
+template <typename T>
+hipError_t launch_trsm_persistent(int n, int m, const T* A, int64_t lda, bool trans, bool lower_op,
+                                  const T* Dinv, bool dinv_t, T* B, int64_t ldb, int32_t* ticket,
+                                  int32_t* flags, int32_t epoch, int32_t* err, int64_t spin_limit,
+                                  hipStream_t s);
+
 // k_util.hip
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t count, hipStream_t s);
 template <typename T>

@@ -62,6 +62,17 @@ def _declare(lib):
         f("dev_solvec").argtypes = [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _err]
 
 
+def _share_torch_runtime():
+    """torch wheels carry their own HIP runtime under the same SONAME
+    (libamdhip64.so.7).  Loading torch first makes libllh.so bind to that one,
+    so a process never holds two HIP runtimes; without torch the system
+    runtime is used."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Loads libllh.so; raises if it has not been built (no fallback exists)."""
     global _lib
@@ -69,6 +80,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C linfa-linalg_amd` "
                                "or __graft_entry__.build()")
+        _share_torch_runtime()
         _lib = ctypes.CDLL(LIB_PATH)
         _declare(_lib)
         try:

@@ -255,3 +255,49 @@ def test_property_solve_triangular(uplo, n, m, data):
     assert np.abs(out - x).max() <= 1e-4
     out2 = llh.solve_triangular_inplace(a, np.array(b), uplo)
     assert np.abs(out2 - x).max() <= 1e-4
+
+
+# --- device-resident forms (llh_dev_*) through torch tensors ----------------------
+
+@pytest.mark.parametrize("n,m", [(100, 1), (1000, 16), (2000, 17), (1500, 64), (700, 65)])
+@pytest.mark.parametrize("uplo,trans", [(L, False), (U, False), (L, True), (U, True)])
+def test_dev_solve_triangular(n, m, uplo, trans):
+    """One-launch solve (m <= 64) and blocked solve (m > 64), every (uplo, trans):
+    op(A) X = B against the oracle's solve of the same op(A)."""
+    import torch
+    from linfa_linalg_hip import device as dev
+    rng = np.random.default_rng(n + m)
+    t = rng.uniform(-1, 1, (n, n)) / np.sqrt(n)
+    t[np.diag_indices(n)] = rng.uniform(1, 2, n)
+    op_lower = uplo == L
+    # op(A)'s triangle is `uplo`; the stored A holds it transposed when trans.
+    opa = np.tril(t) if op_lower else np.triu(t)
+    stored = opa.T.copy() if trans else opa.copy()
+    other = np.triu(np.ones((n, n)), 1) if (op_lower != trans) else np.tril(np.ones((n, n)), -1)
+    stored[other > 0] = np.nan  # never read
+    b = rng.standard_normal((n, m))
+    x_ref = orc.solve_triangular(opa, b, uplo)
+    ta = torch.from_numpy(stored).cuda()
+    tb = torch.from_numpy(b).cuda()
+    dev.solve_triangular(ta, tb, uplo, trans=trans)
+    x = tb.cpu().numpy()
+    assert np.abs(x - x_ref).max() <= 1e-11 * max(1.0, np.abs(x_ref).max())
+
+
+@pytest.mark.parametrize("n", [64, 700, 3000])
+def test_dev_solvec_and_potrf(n):
+    import torch
+    from linfa_linalg_hip import device as dev
+    a = spd(n, 5 * n)
+    b = np.random.default_rng(n).standard_normal((n, 16))
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    dev.solvec(ta, tb)
+    x = tb.cpu().numpy()
+    assert np.linalg.norm(a @ x - b) / (np.linalg.norm(a) * np.linalg.norm(x)) <= 1e-15
+    fac = ta.cpu().numpy()
+    assert backward_error(a, fac) <= 1e-13
+    assert np.array_equal(np.triu(fac, 1), np.triu(a, 1))  # dirty: upper untouched
+    tc = torch.from_numpy(a).cuda()
+    dev.potrf(tc, zero_upper=True)
+    c = tc.cpu().numpy()
+    assert np.all(np.triu(c, 1) == 0) and np.array_equal(c, np.tril(fac))
