@@ -169,6 +169,60 @@ LLH_DEV_DECL(double, d)
 LLH_DEV_DECL(float, s)
 #undef LLH_DEV_DECL
 
+/* ---------------------------------------------------------------------------
+ * Mixed precision with iterative refinement (no counterpart in the reference;
+ * for callers that want an f64-accurate SPD solve at FP32-MFMA speed).
+ *
+ * The factor holds L in the precision `prec`: the trailing updates run in that
+ * precision on MFMA, the 64 x 64 diagonal blocks are factored and inverted in
+ * f64, and solves refine x in f64 against the caller's f64 A:
+ *   x = solve_low(b); repeat { r = b - A x (f64); stop when every column has
+ *   ||r||_inf <= ||x||_inf * ||A||_inf * eps64 * sqrt(n); x += solve_low(r) }.
+ * A is read from its lower triangle only and must stay valid and unchanged
+ * while the factor is used.  A factor is bound to its context's stream.
+ * ------------------------------------------------------------------------- */
+typedef enum llh_precision {
+  LLH_PREC_F64 = 0, /* plain f64 factor, no refinement */
+  LLH_PREC_F32 = 1  /* f32 trailing updates on MFMA, f64 diagonal blocks, f64 refinement */
+} llh_precision;
+
+enum { LLH_FACTOR_FALLBACK_F64 = 1 }; /* on f32 breakdown (pivot <= 0) or stalled refinement, redo in f64 */
+
+typedef struct llh_refine_opts {
+  int32_t max_iter; /* corrections before giving up (default 30 when <= 0) */
+  int32_t flags;    /* LLH_FACTOR_FALLBACK_F64 */
+} llh_refine_opts;
+
+typedef struct llh_refine_info {
+  int32_t iterations;     /* corrections applied */
+  int32_t converged;      /* 1 when the stopping test passed */
+  int32_t fell_back;      /* 1 when the f64 factor produced the result */
+  int32_t precision_used; /* llh_precision of the factor that produced x */
+  double residual;        /* max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf) */
+  double anorm;           /* ||A||_inf */
+} llh_refine_info;
+
+typedef struct llh_factor llh_factor;
+
+/* Factors the device f64 matrix a (n x n, row-major lda, lower triangle read).
+ * With LLH_PREC_F32 and LLH_FACTOR_FALLBACK_F64 in flags, a pivot failure of the
+ * f32 factor falls back to an f64 factor; otherwise it reports
+ * NotPositiveDefinite with the failing row. */
+llh_status llh_factor_create(llh_context* ctx, llh_precision prec, size_t n, const double* a, size_t lda,
+                             int32_t flags, llh_factor** out, llh_error* err);
+/* x (n x m, ldx) = A^-1 b (n x m, ldb), device f64; b and x may not alias. */
+llh_status llh_factor_solve(llh_context* ctx, llh_factor* f, size_t m, const double* b, size_t ldb, double* x,
+                            size_t ldx, const llh_refine_opts* opts, llh_refine_info* info, llh_error* err);
+llh_precision llh_factor_precision(const llh_factor* f);
+size_t llh_factor_device_bytes(const llh_factor* f);
+void llh_factor_destroy(llh_factor* f);
+
+/* One-shot form of the two calls above, with the factor's storage taken from the
+ * context's reusable buffers (no allocation once they have grown). */
+llh_status llh_dev_solve_mixed(llh_context* ctx, llh_precision prec, size_t n, const double* a, size_t lda,
+                               size_t m, const double* b, size_t ldb, double* x, size_t ldx,
+                               const llh_refine_opts* opts, llh_refine_info* info, llh_error* err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -532,6 +532,8 @@ void llh_context_destroy(llh_context* ctx) {
   for (auto& b : ctx->scratch)
     if (b.ptr) (void)hipFree(b.ptr);
   if (ctx->dinv.ptr) (void)hipFree(ctx->dinv.ptr);
+  if (ctx->mixed_l.ptr) (void)hipFree(ctx->mixed_l.ptr);
+  if (ctx->mixed_work.ptr) (void)hipFree(ctx->mixed_work.ptr);
   if (ctx->sync.ptr) (void)hipFree(ctx->sync.ptr);
   if (ctx->d_info) (void)hipFree(ctx->d_info);
   if (ctx->h_info) (void)hipHostFree(ctx->h_info);

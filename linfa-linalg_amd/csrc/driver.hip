@@ -29,7 +29,7 @@ namespace llh {
 // kernel alone, in the reference's exact operation order.
 template <typename T>
 hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, T* dinv, hipStream_t s,
-                         Profiler* prof) {
+                         Profiler* prof, bool diag_f64) {
   const double es = (double)sizeof(T);
   LLH_TRY(launch_fill_i32(info, kNoFail, 1, s));
   if (n <= 0) return hipSuccess;
@@ -44,12 +44,22 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
         const int b = std::min(kNB, k0 + kb - k);
         T* akk = a + (int64_t)k * lda + k;
         const double db = b;
-        LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * (db + 1.0),
-                 (launch_potrf_exact_single<T>(akk, lda, b, info, k, info, s)));
+        if (diag_f64 && sizeof(T) == 4) {
+          LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * (db + 1.0),
+                   (launch_potrf_exact_single<T, double>(akk, lda, b, info, k, info, s)));
+        } else {
+          LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * (db + 1.0),
+                   (launch_potrf_exact_single<T>(akk, lda, b, info, k, info, s)));
+        }
         // inv(L11): serves this panel's solve and, later, the triangular solves.
         T* dk = dinv + (int64_t)(k / kNB) * kNB * kNB;
-        LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * db,
-                 (launch_trtri_blocks<T>(b, akk, lda, false, false, dk, info, s)));
+        if (diag_f64 && sizeof(T) == 4) {
+          LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * db,
+                   (launch_trtri_blocks<T, double>(b, akk, lda, false, false, dk, info, s)));
+        } else {
+          LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * db,
+                   (launch_trtri_blocks<T>(b, akk, lda, false, false, dk, info, s)));
+        }
         const int below = n - k - b;
         if (below <= 0) continue;
         T* panel = akk + (int64_t)b * lda;  // rows k+b.., columns k..k+b
@@ -144,8 +154,9 @@ hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t ld
 }
 
 template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool, double*, hipStream_t,
-                                          Profiler*);
-template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, float*, hipStream_t, Profiler*);
+                                          Profiler*, bool);
+template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, float*, hipStream_t, Profiler*,
+                                         bool);
 template hipError_t trsm_enqueue<double>(bool, bool, int, const double*, int64_t, int, double*, int64_t,
                                          const double*, bool, double*, SolveSync*, hipStream_t, Profiler*);
 template hipError_t trsm_enqueue<float>(bool, bool, int, const float*, int64_t, int, float*, int64_t,

@@ -30,15 +30,16 @@ __device__ __forceinline__ int pk(int n, int j, int k) { return k * n - ((k * (k
 // Returns -1 on success or the first row j whose pivot fails `d <= 0`.  On
 // return W holds every final column (all columns on success, columns < fail
 // otherwise) and Dg the square roots of the passed pivots.
-template <typename T, int NT>
-__device__ int factor_exact(const T* __restrict__ a, int64_t lda, int n, T* W, T* Ap, T* Dg) {
+template <typename TS, typename T, int NT, int NMAX>
+__device__ int factor_exact(const TS* __restrict__ a, int64_t lda, int n, T* W, T* Ap, T* Dg) {
+  constexpr int KG = 8;  // columns whose loads are issued together
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   constexpr int NW = NT / 64;
   const int tot = (n * (n + 1)) >> 1;
   for (int e = tid; e < tot; e += NT) W[e] = T(0);
   for (int j = wave; j < n; j += NW)
-    for (int k = lane; k <= j; k += 64) Ap[pk(n, j, k)] = a[(int64_t)j * lda + k];
+    for (int k = lane; k <= j; k += 64) Ap[pk(n, j, k)] = T(a[(int64_t)j * lda + k]);
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     // Every thread evaluates the pivot identically, so no broadcast is needed.
@@ -52,11 +53,31 @@ __device__ int factor_exact(const T* __restrict__ a, int64_t lda, int n, T* W, T
     }
     __syncthreads();
     // Pending sums: S[j,k'] += L[k',k] * L[j,k] for k < k' <= j (k' == j is d_j).
-    for (int kp = k + 1 + wave; kp < n; kp += NW) {
-      const T lk = W[pk(n, kp, k)];
-      const int base_k = pk(n, kp, k) - kp;   // W[base_k + j]   == L[j, k]
-      const int base_kp = pk(n, kp, kp) - kp; // W[base_kp + j]  == S[j, k']
-      for (int j = kp + lane; j < n; j += 64) W[base_kp + j] = W[base_kp + j] + lk * W[base_k + j];
+    // Thread (lane, wave) owns rows j = lane + 64 r and columns k' = k+1+wave+NW i;
+    // the loads of a group of KG columns are issued together (they never alias
+    // the column-k values read), then the group is updated.
+    const int colk = pk(n, k, k) - k;  // W[colk + i] == L[i, k]
+#pragma unroll
+    for (int r = 0; r < (NMAX + 63) / 64; ++r) {
+      const int j = lane + 64 * r;
+      if (j > k && j < n) {
+        const T ljk = W[colk + j];
+        for (int kp0 = k + 1 + wave; kp0 <= j; kp0 += KG * NW) {
+          T lk[KG], sv[KG];
+#pragma unroll
+          for (int g = 0; g < KG; ++g) {
+            const int kp = kp0 + g * NW;
+            const bool on = kp <= j;
+            lk[g] = on ? W[colk + kp] : T(0);
+            sv[g] = on ? W[pk(n, j, kp)] : T(0);
+          }
+#pragma unroll
+          for (int g = 0; g < KG; ++g) {
+            const int kp = kp0 + g * NW;
+            if (kp <= j) W[pk(n, j, kp)] = sv[g] + lk[g] * ljk;
+          }
+        }
+      }
     }
     __syncthreads();
   }
@@ -67,28 +88,30 @@ __device__ int factor_exact(const T* __restrict__ a, int64_t lda, int n, T* W, T
 // triangle is never touched, which is the dirty form).  On failure at row f it
 // writes rows < f and L[f, 0..f-1], exactly what the reference has written when
 // it returns NotPositiveDefinite.
-template <typename T, int NT>
-__device__ void write_back(T* __restrict__ a, int64_t lda, int n, int fail, const T* W, const T* Dg) {
+template <typename TS, typename T, int NT>
+__device__ void write_back(TS* __restrict__ a, int64_t lda, int n, int fail, const T* W, const T* Dg) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   constexpr int NW = NT / 64;
   const int rows = fail < 0 ? n : fail + 1;
   for (int j = wave; j < rows; j += NW) {
     const int kend = (j == fail) ? j : j + 1;
-    for (int k = lane; k < kend; k += 64) a[(int64_t)j * lda + k] = (k == j) ? Dg[j] : W[pk(n, j, k)];
+    for (int k = lane; k < kend; k += 64) a[(int64_t)j * lda + k] = TS((k == j) ? Dg[j] : W[pk(n, j, k)]);
   }
 }
 
-template <typename T, int NMAX, int NT>
-__global__ __launch_bounds__(NT) void potrf_exact_single(T* a, int64_t lda, int n, int32_t* info,
+// TS is the storage type of `a`, T the arithmetic type (f32 storage with f64
+// arithmetic factors the diagonal blocks of the mixed-precision path).
+template <typename TS, typename T, int NMAX, int NT>
+__global__ __launch_bounds__(NT) void potrf_exact_single(TS* a, int64_t lda, int n, int32_t* info,
                                                          int64_t info_base, const int32_t* gate) {
   if (gate != nullptr && *gate != kNoFail) return;
   __shared__ T W[(NMAX * (NMAX + 1)) / 2];
   __shared__ T Ap[(NMAX * (NMAX + 1)) / 2];
   __shared__ T Dg[NMAX];
-  const int fail = factor_exact<T, NT>(a, lda, n, W, Ap, Dg);
+  const int fail = factor_exact<TS, T, NT, NMAX>(a, lda, n, W, Ap, Dg);
   __syncthreads();
-  write_back<T, NT>(a, lda, n, fail, W, Dg);
+  write_back<TS, T, NT>(a, lda, n, fail, W, Dg);
   if (fail >= 0 && threadIdx.x == 0) atomicMin(info, (int32_t)(info_base + fail));
 }
 
@@ -106,9 +129,9 @@ __global__ __launch_bounds__(NT) void potrf_potrs_exact_batched(const SmallDesc<
   __shared__ T Dg[NMAX];
   __shared__ T Bs[NMAX * MCH];
   __shared__ T Xs[NMAX * MCH];
-  const int fail = factor_exact<T, NT>(ds.a, ds.lda, n, W, Ap, Dg);
+  const int fail = factor_exact<T, T, NT, NMAX>(ds.a, ds.lda, n, W, Ap, Dg);
   __syncthreads();
-  write_back<T, NT>(ds.a, ds.lda, n, fail, W, Dg);
+  write_back<T, T, NT>(ds.a, ds.lda, n, fail, W, Dg);
   if (threadIdx.x == 0) ds.info[0] = fail < 0 ? kNoFail : fail;
   if (fail >= 0 || !do_solve) return;
   const int tid = threadIdx.x;
@@ -152,15 +175,15 @@ __global__ __launch_bounds__(NT) void potrf_potrs_exact_batched(const SmallDesc<
 
 }  // namespace
 
-template <typename T>
-hipError_t launch_potrf_exact_single(T* a, int64_t lda, int n, int32_t* info, int64_t info_base,
+template <typename TS, typename T>
+hipError_t launch_potrf_exact_single(TS* a, int64_t lda, int n, int32_t* info, int64_t info_base,
                                      const int32_t* gate, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (n <= 64) {
-    hipLaunchKernelGGL((potrf_exact_single<T, 64, 256>), dim3(1), dim3(256), 0, s, a, lda, n, info,
+    hipLaunchKernelGGL((potrf_exact_single<TS, T, 64, 256>), dim3(1), dim3(256), 0, s, a, lda, n, info,
                        info_base, gate);
   } else if (n <= small_max<T>()) {
-    hipLaunchKernelGGL((potrf_exact_single<T, small_max_c<T>::value, 256>), dim3(1), dim3(256), 0, s,
+    hipLaunchKernelGGL((potrf_exact_single<TS, T, small_max_c<T>::value, 256>), dim3(1), dim3(256), 0, s,
                        a, lda, n, info, info_base, gate);
   } else {
     return hipErrorInvalidValue;
@@ -187,10 +210,12 @@ hipError_t launch_potrf_potrs_exact_batched(const SmallDesc<T>* descs, int count
   return hipGetLastError();
 }
 
-template hipError_t launch_potrf_exact_single<double>(double*, int64_t, int, int32_t*, int64_t,
-                                                      const int32_t*, hipStream_t);
-template hipError_t launch_potrf_exact_single<float>(float*, int64_t, int, int32_t*, int64_t,
-                                                     const int32_t*, hipStream_t);
+template hipError_t launch_potrf_exact_single<double, double>(double*, int64_t, int, int32_t*, int64_t,
+                                                              const int32_t*, hipStream_t);
+template hipError_t launch_potrf_exact_single<float, float>(float*, int64_t, int, int32_t*, int64_t,
+                                                            const int32_t*, hipStream_t);
+template hipError_t launch_potrf_exact_single<float, double>(float*, int64_t, int, int32_t*, int64_t,
+                                                             const int32_t*, hipStream_t);
 template hipError_t launch_potrf_potrs_exact_batched<double>(const SmallDesc<double>*, int, int, int,
                                                              hipStream_t);
 template hipError_t launch_potrf_potrs_exact_batched<float>(const SmallDesc<float>*, int, int, int,

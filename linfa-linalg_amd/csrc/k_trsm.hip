@@ -116,9 +116,9 @@ __global__ __launch_bounds__(64) void trsv_block_kernel(int b, bool trans, const
 //   [P 0; C Q]^-1 = [P^-1 0; -Q^-1 C P^-1  Q^-1]
 // at 32 and at 64.  The result is written row-major, transposed back when op(A)'s
 // block is upper, with exact zeros outside the triangle.
-template <typename T>
-__global__ __launch_bounds__(256) void trtri64_kernel(int n, const T* __restrict__ A, int64_t lda, bool trans,
-                                                      bool upper, T* __restrict__ out,
+template <typename TS, typename T>
+__global__ __launch_bounds__(256) void trtri64_kernel(int n, const TS* __restrict__ A, int64_t lda, bool trans,
+                                                      bool upper, TS* __restrict__ out,
                                                       const int32_t* __restrict__ gate) {
   if (gate != nullptr && *gate != kNoFail) return;
   constexpr int S = kNB + 1;
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void trtri64_kernel(int n, const T* __restrict
     const int r = flip ? j : i, c = flip ? i : j;
     T v = T(0);
     if (r < b && c < b) {
-      if (c <= r) v = A[(int64_t)(o + i) * lda + (o + j)];
+      if (c <= r) v = T(A[(int64_t)(o + i) * lda + (o + j)]);
     } else if (r == c) {
       v = T(1);
     }
@@ -183,10 +183,10 @@ __global__ __launch_bounds__(256) void trtri64_kernel(int n, const T* __restrict
     }
     __syncthreads();
   }
-  T* dst = out + (int64_t)blockIdx.x * kNB * kNB;
+  TS* dst = out + (int64_t)blockIdx.x * kNB * kNB;
   for (int e = tid; e < kNB * kNB; e += 256) {
     const int r = e / kNB, c = e % kNB;
-    dst[e] = upper ? Xs[c * S + r] : Xs[r * S + c];
+    dst[e] = TS(upper ? Xs[c * S + r] : Xs[r * S + c]);
   }
 }
 
@@ -328,12 +328,12 @@ hipError_t launch_trsv_block(bool upper_op, bool trans, int b, const T* Tm, int6
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_trtri_blocks(int n, const T* A, int64_t lda, bool trans, bool upper, T* out,
+template <typename TS, typename T>
+hipError_t launch_trtri_blocks(int n, const TS* A, int64_t lda, bool trans, bool upper, TS* out,
                                const int32_t* gate, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL((trtri64_kernel<T>), dim3((n + kNB - 1) / kNB), dim3(256), 0, s, n, A, lda, trans, upper,
-                     out, gate);
+  hipLaunchKernelGGL((trtri64_kernel<TS, T>), dim3((n + kNB - 1) / kNB), dim3(256), 0, s, n, A, lda, trans,
+                     upper, out, gate);
   return hipGetLastError();
 }
 
@@ -351,10 +351,12 @@ hipError_t launch_trsm_persistent(int n, int m, const T* A, int64_t lda, bool tr
   return hipGetLastError();
 }
 
-template hipError_t launch_trtri_blocks<double>(int, const double*, int64_t, bool, bool, double*, const int32_t*,
-                                                hipStream_t);
-template hipError_t launch_trtri_blocks<float>(int, const float*, int64_t, bool, bool, float*, const int32_t*,
-                                               hipStream_t);
+template hipError_t launch_trtri_blocks<double, double>(int, const double*, int64_t, bool, bool, double*,
+                                                        const int32_t*, hipStream_t);
+template hipError_t launch_trtri_blocks<float, float>(int, const float*, int64_t, bool, bool, float*,
+                                                      const int32_t*, hipStream_t);
+template hipError_t launch_trtri_blocks<float, double>(int, const float*, int64_t, bool, bool, float*,
+                                                       const int32_t*, hipStream_t);
 template hipError_t launch_trsm_persistent<double>(int, int, const double*, int64_t, bool, bool, const double*,
                                                    bool, double*, int64_t, int32_t*, int32_t*, int32_t, int32_t*,
                                                    int64_t, hipStream_t);

@@ -23,6 +23,8 @@ struct llh_context {
   };
   Buffer scratch[4];  // grow-only device buffers of the plain forms
   Buffer dinv;        // inverses of the 64 x 64 diagonal blocks
+  Buffer mixed_l;     // factor storage of the one-shot mixed-precision solve
+  Buffer mixed_work;  // its residual / correction / norm buffers
   Buffer sync;        // ticket / error / flags of the one-launch solve
   int32_t epoch = 1;
   llh::Profiler prof;  // live kernel timing (off unless enabled)

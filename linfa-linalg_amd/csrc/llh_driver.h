@@ -23,9 +23,11 @@ inline int64_t dinv_elems(int n) { return (int64_t)((n + kNB - 1) / kNB) * kNB *
 // dinv is non-null it receives inv(L_ii) of every 64 x 64 diagonal block (the
 // blocked path produces them on the way; they serve the panel solves and the
 // later triangular solves).
+// diag_f64 (f32 storage only): the diagonal blocks are factored and inverted in
+// f64 arithmetic, the mixed-precision form.
 template <typename T>
 hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, T* dinv, hipStream_t s,
-                         Profiler* prof = nullptr);
+                         Profiler* prof = nullptr, bool diag_f64 = false);
 
 // Enqueues op(A) X = B in place (B row-major n x m); upper_op names the triangle
 // of op(A) and trans != 0 means op(A) = A^T.  dinv, when non-null, holds the

@@ -26,8 +26,10 @@ template <typename T>
 struct small_max_c { static constexpr int value = small_max<T>(); };
 
 // k_small.hip: exact-order LDS Cholesky (bit-identical to the reference order).
-template <typename T>
-hipError_t launch_potrf_exact_single(T* a, int64_t lda, int n, int32_t* info, int64_t info_base,
+// TS is the storage type, T the arithmetic type (T = TS except for the f64
+// diagonal blocks of the mixed-precision factor).
+template <typename TS, typename T = TS>
+hipError_t launch_potrf_exact_single(TS* a, int64_t lda, int n, int32_t* info, int64_t info_base,
                                      const int32_t* gate, hipStream_t s);
 template <typename T>
 hipError_t launch_potrf_potrs_exact_batched(const SmallDesc<T>* descs, int count, int nmax, int do_solve,
@@ -51,8 +53,8 @@ hipError_t launch_trsv_block(bool upper_op, bool trans, int b, const T* Tm, int6
 
 // k_trsm.hip: inverses of the 64 x 64 diagonal blocks of op(A) (n x n), written
 // as consecutive row-major 64 x 64 blocks at out; upper names op(A)'s triangle.
-template <typename T>
-hipError_t launch_trtri_blocks(int n, const T* A, int64_t lda, bool trans, bool upper, T* out,
+template <typename TS, typename T = TS>
+hipError_t launch_trtri_blocks(int n, const TS* A, int64_t lda, bool trans, bool upper, TS* out,
                                const int32_t* gate, hipStream_t s);
 // k_trsm.hip: one-launch blocked solve op(A) X = B (B row-major n x m) using the
 // diagonal-block inverses Dinv (transposed when dinv_t).  ticket/flags/err are
@@ -63,6 +65,16 @@ hipError_t launch_trsm_persistent(int n, int m, const T* A, int64_t lda, bool tr
                                   const T* Dinv, bool dinv_t, T* B, int64_t ldb, int32_t* ticket,
                                   int32_t* flags, int32_t epoch, int32_t* err, int64_t spin_limit,
                                   hipStream_t s);
+
+// k_mixed.hip: conversions, symmetric residual and norms of the refinement.
+hipError_t launch_convert_d2s(int rows, int cols, const double* X, int64_t ldx, float* Y, int64_t ldy,
+                              bool lower_only, hipStream_t s);
+hipError_t launch_axpy_s2d(int rows, int cols, const float* D, int64_t ldd, double* X, int64_t ldx,
+                           hipStream_t s);
+hipError_t launch_symm_residual(int n, int m, const double* A, int64_t lda, const double* X, int64_t ldx,
+                                const double* B, int64_t ldb, double* R, int64_t ldr, hipStream_t s);
+hipError_t launch_symm_abs_rowsum(int n, const double* A, int64_t lda, double* out, hipStream_t s);
+hipError_t launch_colmax(int rows, int cols, const double* X, int64_t ldx, double* out, hipStream_t s);
 
 // k_util.hip
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t count, hipStream_t s);

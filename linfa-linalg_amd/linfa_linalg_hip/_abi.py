@@ -32,6 +32,19 @@ class KernelStats(ctypes.Structure):
                 ("flops", ctypes.c_double), ("bytes", ctypes.c_double)]
 
 
+class RefineOpts(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+class RefineInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("fell_back", ctypes.c_int32), ("precision_used", ctypes.c_int32),
+                ("residual", ctypes.c_double), ("anorm", ctypes.c_double)]
+
+
+PREC = {"f64": 0, "f32": 1}
+FALLBACK_F64 = 1
+
 KERNEL_CLASSES = {"trailing_syrk": 0, "panel_update": 1, "diag_factor": 2, "panel_trsm": 3,
                   "solve_gemm": 4, "solve_diag": 5}
 
@@ -47,6 +60,16 @@ def _declare(lib):
     lib.llh_status_string.restype = ctypes.c_char_p
     lib.llh_context_set_profiling.argtypes = [_vp, _int, _err]
     lib.llh_context_kernel_stats.argtypes = [_vp, _int, ctypes.POINTER(KernelStats), _err]
+    lib.llh_factor_create.argtypes = [_vp, _int, _sz, _vp, _sz, ctypes.c_int32, ctypes.POINTER(_vp), _err]
+    lib.llh_factor_solve.argtypes = [_vp, _vp, _sz, _vp, _sz, _vp, _sz, ctypes.POINTER(RefineOpts),
+                                     ctypes.POINTER(RefineInfo), _err]
+    lib.llh_factor_precision.argtypes = [_vp]
+    lib.llh_factor_device_bytes.argtypes = [_vp]
+    lib.llh_factor_device_bytes.restype = _sz
+    lib.llh_factor_destroy.argtypes = [_vp]
+    lib.llh_factor_destroy.restype = None
+    lib.llh_dev_solve_mixed.argtypes = [_vp, _int, _sz, _vp, _sz, _sz, _vp, _sz, _vp, _sz,
+                                        ctypes.POINTER(RefineOpts), ctypes.POINTER(RefineInfo), _err]
     for s in ("d", "s"):
         f = lambda n: getattr(lib, f"llh_{n}_{s}")  # noqa: E731
         f("cholesky_inplace_dirty").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _err]

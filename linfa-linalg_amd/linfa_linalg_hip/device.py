@@ -91,3 +91,81 @@ def kernel_stats():
         _raise(c._lib.llh_context_kernel_stats(c.handle, cls, ctypes.byref(st), ctypes.byref(err)), err)
         out[name] = (st.launches, st.total_ms, st.flops, st.bytes)
     return out
+
+
+def _info_dict(info):
+    return {"iterations": info.iterations, "converged": bool(info.converged),
+            "fell_back": bool(info.fell_back),
+            "precision_used": {v: k for k, v in _abi.PREC.items()}[info.precision_used],
+            "residual": info.residual, "anorm": info.anorm}
+
+
+def _opts(max_iter, fallback):
+    return _abi.RefineOpts(int(max_iter), _abi.FALLBACK_F64 if fallback else 0)
+
+
+def solve_mixed(a, b, x, prec="f32", max_iter=30, fallback=True):
+    """x = a^-1 b for an f64 SPD a (lower triangle read), factored in `prec`
+    with f64 iterative refinement (llh_dev_solve_mixed).  Returns the
+    refinement report."""
+    for t in (a, b, x):
+        _check(t)
+    n, m = b.shape
+    c = context()
+    err = _abi.LlhError()
+    info = _abi.RefineInfo()
+    opts = _opts(max_iter, fallback)
+    st = c._lib.llh_dev_solve_mixed(c.handle, _abi.PREC[prec], n, ctypes.c_void_p(a.data_ptr()), a.stride(0), m,
+                                    ctypes.c_void_p(b.data_ptr()), b.stride(0), ctypes.c_void_p(x.data_ptr()),
+                                    x.stride(0), ctypes.byref(opts), ctypes.byref(info), ctypes.byref(err))
+    _raise(st, err)
+    return _info_dict(info)
+
+
+class Factor:
+    """A resident factor of an f64 SPD matrix (llh_factor_*): repeated
+    right-hand sides reuse it.  `a` must stay alive and unchanged."""
+
+    def __init__(self, a, prec="f32", fallback=True):
+        _check(a)
+        self._a = a
+        self._c = context()
+        self._ptr = ctypes.c_void_p()
+        err = _abi.LlhError()
+        st = self._c._lib.llh_factor_create(self._c.handle, _abi.PREC[prec], a.shape[0],
+                                            ctypes.c_void_p(a.data_ptr()), a.stride(0),
+                                            _abi.FALLBACK_F64 if fallback else 0, ctypes.byref(self._ptr),
+                                            ctypes.byref(err))
+        _raise(st, err)
+        self.fallback = fallback
+
+    @property
+    def precision(self):
+        return {v: k for k, v in _abi.PREC.items()}[self._c._lib.llh_factor_precision(self._ptr)]
+
+    @property
+    def device_bytes(self):
+        return self._c._lib.llh_factor_device_bytes(self._ptr)
+
+    def solve(self, b, x, max_iter=30):
+        _check(b)
+        _check(x)
+        err = _abi.LlhError()
+        info = _abi.RefineInfo()
+        opts = _opts(max_iter, self.fallback)
+        st = self._c._lib.llh_factor_solve(self._c.handle, self._ptr, b.shape[1], ctypes.c_void_p(b.data_ptr()),
+                                           b.stride(0), ctypes.c_void_p(x.data_ptr()), x.stride(0),
+                                           ctypes.byref(opts), ctypes.byref(info), ctypes.byref(err))
+        _raise(st, err)
+        return _info_dict(info)
+
+    def close(self):
+        if self._ptr:
+            self._c._lib.llh_factor_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
