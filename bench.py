@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -136,6 +137,35 @@ def main():
         log(f"bench: residual {check['residual']:.3e} backward error {check['backward_error']:.3e}")
         del l, res, al
 
+    # FP32-MFMA + f64 iterative refinement on the same A (A is not modified).
+    mixed = None
+    if not args.no_mixed:
+        x32 = torch.zeros(n, m, dtype=torch.float64, device="cuda")
+        log("bench: FP32+IR warmup")
+        info = dev.solve_mixed(a0, rhs0, x32, prec="f32", fallback=True)
+        torch.cuda.synchronize()
+        dev.set_profiling(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            info = dev.solve_mixed(a0, rhs0, x32, prec="f32", fallback=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        stats32 = dev.kernel_stats()
+        dev.set_profiling(False)
+        ms32 = (t1 - t0) * 1e3 / args.steps
+        g32 = solve_flops(n, m) / (ms32 / 1e3) / 1e9
+        cnt32, tot32, fl32, _ = stats32["trailing_syrk"]
+        x64 = copies[-1][1]
+        mixed = {"value": round(g32, 2), "unit": "GFLOP/s", "ms_per_step": round(ms32, 3),
+                 "speedup_vs_f64": round(g32 / gflops, 3), "refinement": info,
+                 "solution_diff_vs_f64_gpu": float((x32 - x64).norm() / x64.norm()),
+                 "syrk_f32_tflops": round(fl32 / (tot32 / 1e3) / 1e12, 2) if tot32 > 0 else None,
+                 "syrk_f32_roofline_frac": round(fl32 / (tot32 / 1e3) / 1e12 / PEAK_TFLOPS["f32"], 4)
+                 if tot32 > 0 else None,
+                 "kernels": {k: round(v[1] / args.steps, 3) for k, v in stats32.items() if v[0]}}
+        log(f"bench: FP32+IR {ms32:.2f} ms/step, {g32:.1f} GFLOP/s, x{g32 / gflops:.2f} vs f64, {info}")
+        del x32
+
     cpu = None
     if not args.no_cpu_baseline:
         log(f"bench: CPU baseline sample n={args.cpu_n}")
@@ -159,6 +189,7 @@ def main():
         "solve_roofline_frac_f64": round(gflops / 1e3 / PEAK_TFLOPS["f64"], 4),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "fp32_ir": mixed,
         "kernels": kernels,
         "check": check,
     }

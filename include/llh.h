@@ -223,6 +223,53 @@ llh_status llh_dev_solve_mixed(llh_context* ctx, llh_precision prec, size_t n, c
                                size_t m, const double* b, size_t ldb, double* x, size_t ldx,
                                const llh_refine_opts* opts, llh_refine_info* info, llh_error* err);
 
+/* ---------------------------------------------------------------------------
+ * Batched solves of many independent SPD systems of uneven sizes (no
+ * counterpart in the reference; for callers with thousands of small systems,
+ * e.g. one per data group).  Each system i is solvec_inplace on its own pair
+ * (A_i, B_i) (cholesky.rs:136-144): A_i is left holding its dirty factor and B_i
+ * its solution.
+ *
+ * Layout: one device buffer holds every A_i, another every B_i.  A_i starts at
+ * element a_off[i] and is row-major n_i x n_i with leading dimension lda[i]
+ * (n_i rounded up to a multiple of 4); B_i starts at b_off[i], row-major
+ * n_i x m_i with leading dimension m_i.  status[i] (device int32) receives -1
+ * (the factor passed) or the first row whose pivot failed `d <= 0`.
+ *
+ * A plan fixes the sizes, buckets them (small systems: one workgroup each in
+ * LDS, in the reference's exact operation order, from a persistent work queue;
+ * larger ones: a blocked factor whose steps run as grouped launches over every
+ * system still active), and captures the whole sequence in one hipGraph that
+ * every llh_batch_solve replays.  llh_batch_autotune times the candidate
+ * bucket bounds and launch shapes on real data and keeps the fastest.
+ * ------------------------------------------------------------------------- */
+typedef struct llh_batch_plan llh_batch_plan;
+
+typedef struct llh_batch_config {
+  int32_t small_limit; /* systems with n <= small_limit take the LDS path (<= 128 f64, <= 176 f32) */
+  int32_t small_waves; /* resident workgroups per CU of the persistent small-system kernels */
+} llh_batch_config;
+
+/* prec: LLH_PREC_F64 (buffers of double) or LLH_PREC_F32 (buffers of float). */
+llh_status llh_batch_plan_create(llh_context* ctx, llh_precision prec, size_t count, const int32_t* n,
+                                 const int32_t* m, const llh_batch_config* config, llh_batch_plan** out,
+                                 llh_error* err);
+/* Element counts of the packed A and B buffers, and the per-system layout. */
+size_t llh_batch_a_elems(const llh_batch_plan* plan);
+size_t llh_batch_b_elems(const llh_batch_plan* plan);
+llh_status llh_batch_layout(const llh_batch_plan* plan, uint64_t* a_off, int32_t* lda, uint64_t* b_off,
+                            llh_error* err);
+llh_batch_config llh_batch_get_config(const llh_batch_plan* plan);
+/* One graph launch: factors every A_i and solves every B_i in place. */
+llh_status llh_batch_solve(llh_context* ctx, llh_batch_plan* plan, void* a, void* b, int32_t* status,
+                           llh_error* err);
+/* Times the candidate configurations on copies of (a, b) (restored after each
+ * trial; a and b are left unchanged) and keeps the fastest; *best_ms receives
+ * its time. */
+llh_status llh_batch_autotune(llh_context* ctx, llh_batch_plan* plan, const void* a, const void* b,
+                              int32_t* status, double* best_ms, llh_error* err);
+void llh_batch_plan_destroy(llh_batch_plan* plan);
+
 #ifdef __cplusplus
 }
 #endif

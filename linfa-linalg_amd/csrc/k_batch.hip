@@ -1,0 +1,211 @@
+// Grouped kernels of the batched blocked solve.  One launch does one step of
+// the blocked algorithm for every matrix of the batch that is still active at
+// that step: each workgroup reads its (matrix, block[, tile]) from a work list
+// the planner built once, so a whole batch of uneven sizes runs as a fixed
+// sequence of launches (captured in one hipGraph).  A matrix whose status word
+// records a failed pivot is skipped by every later launch.
+#include "llh_common.h"
+#include "k_gemm_impl.h"
+#include "k_trsm_impl.h"
+#include "llh_kernels.h"
+
+namespace llh {
+namespace {
+
+template <typename T>
+__device__ __forceinline__ bool failed(const BatchBase<T>& bb, const BatchItem& it) {
+  return bb.status[it.id] != kNoFail;
+}
+
+// inv(L_kk) of the block at (k0, k0) into the matrix's inverse slot k0 / 64.
+template <typename T>
+__global__ __launch_bounds__(256) void grp_trtri_kernel(const BatchBase<T>* __restrict__ base,
+                                                        const BatchItem* __restrict__ items,
+                                                        const StepWork* __restrict__ work, int count) {
+  if ((int)blockIdx.x >= count) return;
+  const StepWork w = work[blockIdx.x];
+  const BatchItem it = items[w.item];
+  const BatchBase<T> bb = *base;
+  if (failed(bb, it)) return;
+  __shared__ T Ms[kNB * (kNB + 1)];
+  __shared__ T Xs[kNB * (kNB + 1)];
+  __shared__ T Ts[kNB * (kNB + 1)];
+  const T* a = bb.a + it.a_off + (int64_t)w.k0 * it.lda + w.k0;
+  trtri64_block<T, T>(min(kNB, it.n - w.k0), a, it.lda, false, false,
+                      bb.linv + it.linv_off + (int64_t)(w.k0 / kNB) * kNB * kNB, Ms, Xs, Ts);
+}
+
+// Panel tile: L21[m0.., :] = A21[m0.., :] * inv(L11)^T, in place (the tile holds
+// all b <= 64 columns of its rows, so the rows it reads are the rows it writes).
+template <typename T>
+__global__ __launch_bounds__(256) void grp_panel_kernel(const BatchBase<T>* __restrict__ base,
+                                                        const BatchItem* __restrict__ items,
+                                                        const TileWork* __restrict__ work, int count) {
+  if ((int)blockIdx.x >= count) return;
+  const TileWork w = work[blockIdx.x];
+  const BatchItem it = items[w.item];
+  const BatchBase<T> bb = *base;
+  if (failed(bb, it)) return;
+  const int b = min(kNB, it.n - w.k0), below = it.n - w.k0 - b;
+  T* p = bb.a + it.a_off + (int64_t)(w.k0 + b) * it.lda + w.k0;
+  const T* li = bb.linv + it.linv_off + (int64_t)(w.k0 / kNB) * kNB * kNB;
+  nt_tile<T, 64, 64, 2, 2, false, true>(below, b, b, T(1), p, it.lda, li, kNB, T(0), p, it.lda, w.m0, 0);
+}
+
+// Trailing tile: A22[m0.., n0..] -= L21 L21^T (lower part).
+template <typename T>
+__global__ __launch_bounds__(256) void grp_syrk_kernel(const BatchBase<T>* __restrict__ base,
+                                                       const BatchItem* __restrict__ items,
+                                                       const TileWork* __restrict__ work, int count) {
+  if ((int)blockIdx.x >= count) return;
+  const TileWork w = work[blockIdx.x];
+  const BatchItem it = items[w.item];
+  const BatchBase<T> bb = *base;
+  if (failed(bb, it)) return;
+  const int b = min(kNB, it.n - w.k0), m2 = it.n - w.k0 - b;
+  T* p = bb.a + it.a_off + (int64_t)(w.k0 + b) * it.lda + w.k0;
+  nt_tile<T, 64, 64, 2, 2, true, true>(m2, m2, b, T(-1), p, it.lda, p, it.lda, T(1), p + b, it.lda, w.m0, w.n0);
+}
+
+// Forward-solve step of block k0: Y = inv(L_kk) B_k, then B_rest -= L21 Y.
+// B columns are processed 8 at a time; thread (r, c) owns row r of the block.
+template <typename T>
+__global__ __launch_bounds__(256) void grp_fwd_kernel(const BatchBase<T>* __restrict__ base,
+                                                      const BatchItem* __restrict__ items,
+                                                      const StepWork* __restrict__ work, int count) {
+  constexpr int MC = 8, S = kNB + 1;
+  if ((int)blockIdx.x >= count) return;
+  const StepWork w = work[blockIdx.x];
+  const BatchItem it = items[w.item];
+  const BatchBase<T> bb = *base;
+  if (failed(bb, it)) return;
+  __shared__ T Li[kNB * S];
+  __shared__ T Ys[kNB * MC];
+  const int tid = threadIdx.x, n = it.n, k0 = w.k0, b = min(kNB, n - k0);
+  const T* li = bb.linv + it.linv_off + (int64_t)(k0 / kNB) * kNB * kNB;
+  const T* a = bb.a + it.a_off;
+  T* bm = bb.b + it.b_off;
+  for (int e = tid; e < kNB * kNB; e += 256) Li[(e / kNB) * S + e % kNB] = li[e];
+  for (int c0 = 0; c0 < it.m; c0 += MC) {
+    const int mc = min(MC, it.m - c0);
+    __syncthreads();
+    for (int e = tid; e < kNB * MC; e += 256) {
+      const int r = e / MC, c = e % MC;
+      Ys[e] = (r < b && c < mc) ? bm[(int64_t)(k0 + r) * it.m + c0 + c] : T(0);
+    }
+    __syncthreads();
+    // Y = Li * Bk (thread: row r, columns c, c + 4)
+    const int r = tid >> 2, c = tid & 3;
+    T y0 = T(0), y1 = T(0);
+    for (int k = 0; k <= r && k < b; ++k) {
+      const T l = Li[r * S + k];
+      y0 += l * Ys[k * MC + c];
+      y1 += l * Ys[k * MC + c + 4];
+    }
+    __syncthreads();
+    Ys[r * MC + c] = y0;
+    Ys[r * MC + c + 4] = y1;
+    if (r < b) {
+      if (c < mc) bm[(int64_t)(k0 + r) * it.m + c0 + c] = y0;
+      if (c + 4 < mc) bm[(int64_t)(k0 + r) * it.m + c0 + c + 4] = y1;
+    }
+    __syncthreads();
+    // B[k0+b+i, :] -= L[k0+b+i, k0..k0+b] * Y
+    const int below = n - k0 - b;
+    for (int e = tid; e < below * MC; e += 256) {
+      const int i = e / MC, cc = e % MC;
+      if (cc >= mc) continue;
+      const T* lr = a + (int64_t)(k0 + b + i) * it.lda + k0;
+      T s = T(0);
+      for (int k = 0; k < b; ++k) s += lr[k] * Ys[k * MC + cc];
+      bm[(int64_t)(k0 + b + i) * it.m + c0 + cc] -= s;
+    }
+  }
+}
+
+// Back-solve step of block k0 (op = L^T): X = inv(L_kk)^T Y_k, then
+// B[0:k0] -= L[k0:k0+b, 0:k0]^T X.
+template <typename T>
+__global__ __launch_bounds__(256) void grp_bwd_kernel(const BatchBase<T>* __restrict__ base,
+                                                      const BatchItem* __restrict__ items,
+                                                      const StepWork* __restrict__ work, int count) {
+  constexpr int MC = 8, S = kNB + 1;
+  if ((int)blockIdx.x >= count) return;
+  const StepWork w = work[blockIdx.x];
+  const BatchItem it = items[w.item];
+  const BatchBase<T> bb = *base;
+  if (failed(bb, it)) return;
+  __shared__ T Li[kNB * S];
+  __shared__ T Ys[kNB * MC];
+  const int tid = threadIdx.x, k0 = w.k0, b = min(kNB, it.n - k0);
+  const T* li = bb.linv + it.linv_off + (int64_t)(k0 / kNB) * kNB * kNB;
+  const T* a = bb.a + it.a_off;
+  T* bm = bb.b + it.b_off;
+  for (int e = tid; e < kNB * kNB; e += 256) Li[(e / kNB) * S + e % kNB] = li[e];
+  for (int c0 = 0; c0 < it.m; c0 += MC) {
+    const int mc = min(MC, it.m - c0);
+    __syncthreads();
+    for (int e = tid; e < kNB * MC; e += 256) {
+      const int r = e / MC, c = e % MC;
+      Ys[e] = (r < b && c < mc) ? bm[(int64_t)(k0 + r) * it.m + c0 + c] : T(0);
+    }
+    __syncthreads();
+    // X = Li^T * Yk: X[r] = sum_{k >= r} Li[k][r] Y[k]
+    const int r = tid >> 2, c = tid & 3;
+    T x0 = T(0), x1 = T(0);
+    for (int k = r; k < b; ++k) {
+      const T l = Li[k * S + r];
+      x0 += l * Ys[k * MC + c];
+      x1 += l * Ys[k * MC + c + 4];
+    }
+    __syncthreads();
+    Ys[r * MC + c] = x0;
+    Ys[r * MC + c + 4] = x1;
+    if (r < b) {
+      if (c < mc) bm[(int64_t)(k0 + r) * it.m + c0 + c] = x0;
+      if (c + 4 < mc) bm[(int64_t)(k0 + r) * it.m + c0 + c + 4] = x1;
+    }
+    __syncthreads();
+    // B[i, :] -= sum_k L[k0+k][i] X[k] for i < k0
+    for (int e = tid; e < k0 * MC; e += 256) {
+      const int i = e / MC, cc = e % MC;
+      if (cc >= mc) continue;
+      T s = T(0);
+      for (int k = 0; k < b; ++k) s += a[(int64_t)(k0 + k) * it.lda + i] * Ys[k * MC + cc];
+      bm[(int64_t)i * it.m + c0 + cc] -= s;
+    }
+  }
+}
+
+__global__ void fill_status_kernel(int32_t* st, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) st[i] = kNoFail;
+}
+
+}  // namespace
+
+#define LLH_GRP_LAUNCH(name, kernel, WorkT)                                                             \
+  template <typename T>                                                                                \
+  hipError_t name(const BatchBase<T>* base, const BatchItem* items, const WorkT* work, int count,        \
+                  hipStream_t s) {                                                                     \
+    if (count <= 0) return hipSuccess;                                                                 \
+    hipLaunchKernelGGL((kernel<T>), dim3(count), dim3(256), 0, s, base, items, work, count);            \
+    return hipGetLastError();                                                                          \
+  }                                                                                                    \
+  template hipError_t name<double>(const BatchBase<double>*, const BatchItem*, const WorkT*, int,       \
+                                   hipStream_t);                                                       \
+  template hipError_t name<float>(const BatchBase<float>*, const BatchItem*, const WorkT*, int, hipStream_t);
+
+LLH_GRP_LAUNCH(launch_grp_trtri, grp_trtri_kernel, StepWork)
+LLH_GRP_LAUNCH(launch_grp_panel, grp_panel_kernel, TileWork)
+LLH_GRP_LAUNCH(launch_grp_syrk, grp_syrk_kernel, TileWork)
+LLH_GRP_LAUNCH(launch_grp_fwd, grp_fwd_kernel, StepWork)
+LLH_GRP_LAUNCH(launch_grp_bwd, grp_bwd_kernel, StepWork)
+
+hipError_t launch_fill_status(int32_t* status, int64_t count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_status_kernel, dim3((count + 255) / 256), dim3(256), 0, s, status, count);
+  return hipGetLastError();
+}
+
+}  // namespace llh

@@ -1,0 +1,164 @@
+// Device code shared by the MFMA GEMM kernels (k_gemm.hip) and the grouped
+// batch kernels (k_batch.hip): the MFMA wrappers and one NT tile,
+//   C[m0.., n0..] = beta * C + alpha * A * B^T  (TRI: only row >= col),
+// with both operands k-contiguous.
+//
+// MFMA lane layouts (measured on gfx950):
+//   A operand: lane l holds op(A)[l%16][k0 + l/16]
+//   B operand: lane l holds op(B)[k0 + l/16][l%16]
+//   f64 result reg i: C[4*i + l/16][l%16];  f32 result reg i: C[4*(l/16) + i][l%16]
+#pragma once
+
+#include "llh_common.h"
+
+namespace llh {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc_t = v4d;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int i) { return 4 * i + (lane >> 4); }
+};
+template <>
+struct Mfma<float> {
+  using acc_t = v4f;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
+};
+
+// One BM x BN tile of C = beta*C + alpha*A*B^T on WM x WN waves.
+//
+// Slabs of SK k are staged in double-buffered LDS, row-major ([row][k]) with
+// the row stride padded to 16 mod 128 bytes, so the 8 lanes of one 128-bit LDS
+// access phase read 8 rows from 8 different bank groups.  Each lane's MFMA
+// fragment is KPL = 16/sizeof(T) consecutive k of one row (one 128-bit read):
+// within a sub-chunk of 4*KPL k, MFMA s takes k = sub + KPL*(lane/16) + s from
+// every lane, for A and B alike, so the KPL MFMAs of a sub-chunk together sum all
+// of its k.  One barrier per slab: slab t+1 is loaded into registers while slab
+// t is multiplied, then written to the other LDS buffer.
+//
+// Rows beyond M (or N) are clamped to the last valid row, so interior loads
+// carry no bounds checks; their products land in rows the epilogue discards.
+// Only a final partial slab of k is loaded with masking.  A may alias C only when
+// each tile's rows of A are its own rows of C (every read precedes the writes).
+template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool VEC>
+__device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const T* A, int64_t lda,
+                                        const T* __restrict__ B, int64_t ldb, T beta, T* C, int64_t ldc,
+                                        int m0, int n0) {
+  constexpr int NTHR = WM * WN * 64;
+  constexpr int TM = BM / WM, TN = BN / WN;
+  constexpr int FM = TM / 16, FN = TN / 16;
+  constexpr int KPL = 16 / sizeof(T);             // k per lane per 128-bit read
+  constexpr int SUB = 4 * KPL;                    // k covered by KPL MFMAs
+  constexpr int SK = 2 * SUB;                     // k per slab
+  constexpr int LDK = SK + KPL;                   // padded LDS row stride (elements)
+  constexpr int LPR = SK / KPL;                   // lanes per slab row
+  constexpr int RPP = NTHR / LPR;                 // rows per load pass
+  constexpr int PA = BM / RPP, PB = BN / RPP;     // load passes
+  static_assert(BM % RPP == 0 && BN % RPP == 0, "tile rows must divide the load passes");
+  using MF = Mfma<T>;
+  using acc_t = typename MF::acc_t;
+  typedef T vec_t __attribute__((ext_vector_type(KPL)));
+
+  __shared__ __attribute__((aligned(16))) T As[2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) T Bs[2][BN * LDK];
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm0 = (wave / WN) * TM, wn0 = (wave % WN) * TN;
+  const int lrow = t / LPR, lk = (t % LPR) * KPL;
+
+  // Clamped source rows of this thread's loads.
+  const T* pa[PA];
+  const T* pb[PB];
+#pragma unroll
+  for (int p = 0; p < PA; ++p) pa[p] = A + (int64_t)min(m0 + lrow + p * RPP, M - 1) * lda + lk;
+#pragma unroll
+  for (int p = 0; p < PB; ++p) pb[p] = B + (int64_t)min(n0 + lrow + p * RPP, N - 1) * ldb + lk;
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  vec_t ra[PA], rb[PB];
+  auto load = [&](int k0) {
+    if (VEC && k0 + SK <= K) {
+#pragma unroll
+      for (int p = 0; p < PA; ++p) ra[p] = *reinterpret_cast<const vec_t*>(pa[p] + k0);
+#pragma unroll
+      for (int p = 0; p < PB; ++p) rb[p] = *reinterpret_cast<const vec_t*>(pb[p] + k0);
+    } else {
+#pragma unroll
+      for (int p = 0; p < PA; ++p)
+#pragma unroll
+        for (int e = 0; e < KPL; ++e) ra[p][e] = (k0 + lk + e < K) ? pa[p][k0 + e] : T(0);
+#pragma unroll
+      for (int p = 0; p < PB; ++p)
+#pragma unroll
+        for (int e = 0; e < KPL; ++e) rb[p][e] = (k0 + lk + e < K) ? pb[p][k0 + e] : T(0);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < PA; ++p) *reinterpret_cast<vec_t*>(&As[buf][(lrow + p * RPP) * LDK + lk]) = ra[p];
+#pragma unroll
+    for (int p = 0; p < PB; ++p) *reinterpret_cast<vec_t*>(&Bs[buf][(lrow + p * RPP) * LDK + lk]) = rb[p];
+  };
+
+  const int nslab = (K + SK - 1) / SK;
+  load(0);
+  store(0);
+  __syncthreads();
+  const int fr = lane & 15, fq = (lane >> 4) * KPL;
+  for (int sl = 0; sl < nslab; ++sl) {
+    const int cur = sl & 1;
+    const bool more = sl + 1 < nslab;
+    if (more) load((sl + 1) * SK);
+    const T* as = As[cur];
+    const T* bs = Bs[cur];
+#pragma unroll
+    for (int c = 0; c < SK; c += SUB) {
+      vec_t af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = *reinterpret_cast<const vec_t*>(&as[(wm0 + i * 16 + fr) * LDK + c + fq]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const vec_t*>(&bs[(wn0 + j * 16 + fr) * LDK + c + fq]);
+#pragma unroll
+      for (int s = 0; s < KPL; ++s)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[i][s], bf[j][s], acc[i][j]);
+    }
+    if (more) store(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = m0 + wm0 + i * 16 + MF::row(lane, r);
+        const int gj = n0 + wn0 + j * 16 + fr;
+        if (gi < M && gj < N && (!TRI || gi >= gj)) {
+          T* c = C + (int64_t)gi * ldc + gj;
+          const T v = alpha * acc[i][j][r];
+          *c = (beta == T(0)) ? v : beta * *c + v;
+        }
+      }
+}
+
+}  // namespace llh

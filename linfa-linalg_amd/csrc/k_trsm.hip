@@ -14,6 +14,7 @@
 // the column), restricted to b <= 64 rows.
 #include "llh_common.h"
 #include "llh_kernels.h"
+#include "k_trsm_impl.h"
 
 namespace llh {
 namespace {
@@ -106,88 +107,18 @@ __global__ __launch_bounds__(64) void trsv_block_kernel(int b, bool trans, const
     if (r < b) Y[(int64_t)r * ldy + col] = y[r];
 }
 
-// ---------------------------------------------------------------------------
-// trtri64: inverses of the 64 x 64 diagonal blocks of op(A) (the last block may
-// be smaller; it is padded with an identity tail).  One workgroup per block.
-// The block is copied to LDS in lower-triangular form M (op(A)'s block, or its
-// transpose when that block is upper triangular); inv(M) is built by recursive
-// doubling: the four 16 x 16 diagonal blocks are inverted by substitution (one
-// thread per column), then pairs are combined with
-//   [P 0; C Q]^-1 = [P^-1 0; -Q^-1 C P^-1  Q^-1]
-// at 32 and at 64.  The result is written row-major, transposed back when op(A)'s
-// block is upper, with exact zeros outside the triangle.
+// trtri64: one workgroup per diagonal block (trtri64_block, k_trsm_impl.h).
 template <typename TS, typename T>
 __global__ __launch_bounds__(256) void trtri64_kernel(int n, const TS* __restrict__ A, int64_t lda, bool trans,
                                                       bool upper, TS* __restrict__ out,
                                                       const int32_t* __restrict__ gate) {
   if (gate != nullptr && *gate != kNoFail) return;
-  constexpr int S = kNB + 1;
-  __shared__ T Ms[kNB * S];
-  __shared__ T Xs[kNB * S];
-  __shared__ T Ts[kNB * S];
-  const int tid = threadIdx.x;
+  __shared__ T Ms[kNB * (kNB + 1)];
+  __shared__ T Xs[kNB * (kNB + 1)];
+  __shared__ T Ts[kNB * (kNB + 1)];
   const int o = blockIdx.x * kNB;
-  const int b = min(kNB, n - o);
-  // Element (r, c) of M, r >= c: op(A)[o+r, o+c] (lower) or op(A)[o+c, o+r] (upper);
-  // with op(A) = A^T when trans, M[r][c] is A[o+c][o+r] exactly when upper != trans.
-  const bool flip = (upper != trans);
-  for (int e = tid; e < kNB * kNB; e += 256) {
-    const int i = e / kNB, j = e % kNB;  // coalesced over j in A's storage
-    const int r = flip ? j : i, c = flip ? i : j;
-    T v = T(0);
-    if (r < b && c < b) {
-      if (c <= r) v = T(A[(int64_t)(o + i) * lda + (o + j)]);
-    } else if (r == c) {
-      v = T(1);
-    }
-    Ms[r * S + c] = v;
-    Xs[r * S + c] = T(0);
-  }
-  __syncthreads();
-  // 16 x 16 diagonal blocks: thread (blk, c) builds column c of inv(D_blk).
-  if (tid < kNB) {
-    const int base = (tid >> 4) * 16, c = tid & 15;
-    T x[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      T s = T(0);
-#pragma unroll
-      for (int p = 0; p < r; ++p) s += (p >= c) ? Ms[(base + r) * S + base + p] * x[p] : T(0);
-      const T d = Ms[(base + r) * S + base + r];
-      x[r] = (r < c) ? T(0) : (r == c ? T(1) / d : -s / d);
-    }
-#pragma unroll
-    for (int r = 0; r < 16; ++r) Xs[(base + r) * S + base + c] = x[r];
-  }
-  __syncthreads();
-  // Combine at h = 16 (two pairs) and h = 32 (one pair).
-#pragma unroll
-  for (int h = 16; h <= 32; h *= 2) {
-    const int pairs = kNB / (2 * h);
-    // T = C * P^-1, C = M[base+h.., base..base+h), P^-1 = X[base.., base..]
-    for (int e = tid; e < pairs * h * h; e += 256) {
-      const int pr = e / (h * h), rem = e % (h * h), i = rem / h, j = rem % h;
-      const int base = pr * 2 * h;
-      T s = T(0);
-      for (int k = j; k < h; ++k) s += Ms[(base + h + i) * S + base + k] * Xs[(base + k) * S + base + j];
-      Ts[(base + h + i) * S + base + j] = s;
-    }
-    __syncthreads();
-    // X21 = -Q^-1 * T, Q^-1 = X[base+h.., base+h..]
-    for (int e = tid; e < pairs * h * h; e += 256) {
-      const int pr = e / (h * h), rem = e % (h * h), i = rem / h, j = rem % h;
-      const int base = pr * 2 * h;
-      T s = T(0);
-      for (int k = 0; k <= i; ++k) s += Xs[(base + h + i) * S + base + h + k] * Ts[(base + h + k) * S + base + j];
-      Xs[(base + h + i) * S + base + j] = -s;
-    }
-    __syncthreads();
-  }
-  TS* dst = out + (int64_t)blockIdx.x * kNB * kNB;
-  for (int e = tid; e < kNB * kNB; e += 256) {
-    const int r = e / kNB, c = e % kNB;
-    dst[e] = TS(upper ? Xs[c * S + r] : Xs[r * S + c]);
-  }
+  trtri64_block<TS, T>(min(kNB, n - o), A + (int64_t)o * lda + o, lda, trans, upper,
+                       out + (int64_t)blockIdx.x * kNB * kNB, Ms, Xs, Ts);
 }
 
 // ---------------------------------------------------------------------------
@@ -237,31 +168,50 @@ __global__ __launch_bounds__(256) void trsm_persistent_kernel(int n, int m, cons
   for (int e = 0; e < 4; ++e)
     acc[e] = (r < bw && cc + e < mc) ? B[(int64_t)(w0 + r) * ldb + c0 + cc + e] : T(0);
 
+  // op(A)[w0+r, j0+k] is A[w0+r][j0+k], or A[j0+k][w0+r] when trans.  Block s
+  // of the walk is prefetched into registers one step ahead (it does not depend
+  // on any flag), element e = tid + 256 g being (i, jj) = (e / 64, e % 64) of the
+  // stored tile, read coalesced over jj.
+  T pre[16];
+  auto fetch = [&](int s) {
+    const int j = lower_op ? s : nb - 1 - s;
+    const int j0 = j * kNB, bj = min(kNB, n - j0);
+#pragma unroll
+    for (int g = 0; g < 16; ++g) {
+      const int e = tid + 256 * g, i = e / kNB, jj = e % kNB;
+      const int rr = trans ? jj : i, kk = trans ? i : jj;
+      pre[g] = (rr < bw && kk < bj)
+                   ? (trans ? A[(int64_t)(j0 + i) * lda + w0 + jj] : A[(int64_t)(w0 + i) * lda + j0 + jj])
+                   : T(0);
+    }
+  };
+  if (p > 0) fetch(0);
   for (int s = 0; s < p; ++s) {
     const int j = lower_op ? s : nb - 1 - s;
     const int j0 = j * kNB, bj = min(kNB, n - j0);
-    // op(A)[w0+r, j0+k]: A[w0+r][j0+k], or A[j0+k][w0+r] when trans.
-    for (int e = tid; e < kNB * kNB; e += 256) {
-      const int i = e / kNB, jj = e % kNB;  // coalesced over jj
-      const int rr = trans ? jj : i, kk = trans ? i : jj;
-      T v = T(0);
-      if (rr < bw && kk < bj) v = trans ? A[(int64_t)(j0 + i) * lda + w0 + jj] : A[(int64_t)(w0 + i) * lda + j0 + jj];
-      As[rr * S + kk] = v;
+#pragma unroll
+    for (int g = 0; g < 16; ++g) {
+      const int e = tid + 256 * g, i = e / kNB, jj = e % kNB;
+      As[(trans ? jj : i) * S + (trans ? i : jj)] = pre[g];
     }
+    if (s + 1 < p) fetch(s + 1);
     if (tid == 0) {
+      // Poll with relaxed loads (no cache maintenance per poll); one acquire
+      // fence once the flag is seen.
       int64_t it = 0;
-      while (__hip_atomic_load(&flags[j * nq + q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+      while (__hip_atomic_load(&flags[j * nq + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
         if (++it > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
           atomicExch(err, 1);
           s_abort = 1;
           break;
         }
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(1);
       }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
     if (s_abort) return;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     for (int e = tid; e < kNB * MC; e += 256) {
       const int k = e / MC, c = e % MC;
       Ys[k * SC + c] = (k < bj && c < mc) ? __builtin_nontemporal_load(&B[(int64_t)(j0 + k) * ldb + c0 + c]) : T(0);

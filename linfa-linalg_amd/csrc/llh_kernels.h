@@ -20,6 +20,41 @@ struct SmallDesc {
   int32_t* info;
 };
 
+// Batched solves (k_batch.hip, k_small.hip).  Matrix i of a batch: A at
+// base.a + a_off (row-major n x n, ld = lda), B at base.b + b_off (row-major
+// n x m, ld = m), inverses of its 64 x 64 diagonal blocks at base.linv + linv_off,
+// status word base.status[id].  The base pointers live in device memory so a
+// captured graph can be replayed on other buffers.
+template <typename T>
+struct BatchBase {
+  T* a;
+  T* b;
+  T* linv;
+  int32_t* status;
+  int32_t* counters;  // work-queue counters of the persistent kernels
+};
+struct BatchItem {
+  int64_t a_off;
+  int64_t b_off;
+  int64_t linv_off;
+  int32_t n;
+  int32_t m;
+  int32_t lda;
+  int32_t id;
+};
+// Work of one grouped launch: matrix item and block start k0; tiles add the
+// tile origin (m0, n0) inside the block's panel or trailing matrix.
+struct StepWork {
+  int32_t item;
+  int32_t k0;
+};
+struct TileWork {
+  int32_t item;
+  int32_t k0;
+  int32_t m0;
+  int32_t n0;
+};
+
 template <typename T>
 constexpr int small_max() { return sizeof(T) == 8 ? kSmallMaxD : kSmallMaxS; }
 template <typename T>
@@ -34,6 +69,33 @@ hipError_t launch_potrf_exact_single(TS* a, int64_t lda, int n, int32_t* info, i
 template <typename T>
 hipError_t launch_potrf_potrs_exact_batched(const SmallDesc<T>* descs, int count, int nmax, int do_solve,
                                             hipStream_t s);
+
+// k_small.hip: persistent exact-order factor + solve over items[0..count),
+// taken from a work-queue counter (base.counters[counter]); nmax <= small_max.
+template <typename T>
+hipError_t launch_small_batch(const BatchBase<T>* base, const BatchItem* items, int count, int nmax, int grid,
+                              int counter, hipStream_t s);
+// k_small.hip: grouped diagonal-block factor of step work (one workgroup each).
+template <typename T>
+hipError_t launch_grp_diag(const BatchBase<T>* base, const BatchItem* items, const StepWork* work, int count,
+                           hipStream_t s);
+// k_batch.hip: grouped steps of the blocked batch factor and solves.
+template <typename T>
+hipError_t launch_grp_trtri(const BatchBase<T>* base, const BatchItem* items, const StepWork* work, int count,
+                            hipStream_t s);
+template <typename T>
+hipError_t launch_grp_panel(const BatchBase<T>* base, const BatchItem* items, const TileWork* work, int count,
+                            hipStream_t s);
+template <typename T>
+hipError_t launch_grp_syrk(const BatchBase<T>* base, const BatchItem* items, const TileWork* work, int count,
+                           hipStream_t s);
+template <typename T>
+hipError_t launch_grp_fwd(const BatchBase<T>* base, const BatchItem* items, const StepWork* work, int count,
+                          hipStream_t s);
+template <typename T>
+hipError_t launch_grp_bwd(const BatchBase<T>* base, const BatchItem* items, const StepWork* work, int count,
+                          hipStream_t s);
+hipError_t launch_fill_status(int32_t* status, int64_t count, hipStream_t s);
 
 // k_gemm.hip: C = beta*C + alpha*op(A)*op(B), optionally lower-triangle only.
 template <typename T>

@@ -42,6 +42,10 @@ class RefineInfo(ctypes.Structure):
                 ("residual", ctypes.c_double), ("anorm", ctypes.c_double)]
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("small_limit", ctypes.c_int32), ("small_waves", ctypes.c_int32)]
+
+
 PREC = {"f64": 0, "f32": 1}
 FALLBACK_F64 = 1
 
@@ -70,6 +74,20 @@ def _declare(lib):
     lib.llh_factor_destroy.restype = None
     lib.llh_dev_solve_mixed.argtypes = [_vp, _int, _sz, _vp, _sz, _sz, _vp, _sz, _vp, _sz,
                                         ctypes.POINTER(RefineOpts), ctypes.POINTER(RefineInfo), _err]
+    i32p, u64p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint64)
+    lib.llh_batch_plan_create.argtypes = [_vp, _int, _sz, i32p, i32p, ctypes.POINTER(BatchConfig),
+                                          ctypes.POINTER(_vp), _err]
+    lib.llh_batch_a_elems.argtypes = [_vp]
+    lib.llh_batch_a_elems.restype = _sz
+    lib.llh_batch_b_elems.argtypes = [_vp]
+    lib.llh_batch_b_elems.restype = _sz
+    lib.llh_batch_layout.argtypes = [_vp, u64p, i32p, u64p, _err]
+    lib.llh_batch_get_config.argtypes = [_vp]
+    lib.llh_batch_get_config.restype = BatchConfig
+    lib.llh_batch_solve.argtypes = [_vp, _vp, _vp, _vp, _vp, _err]
+    lib.llh_batch_autotune.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_double), _err]
+    lib.llh_batch_plan_destroy.argtypes = [_vp]
+    lib.llh_batch_plan_destroy.restype = None
     for s in ("d", "s"):
         f = lambda n: getattr(lib, f"llh_{n}_{s}")  # noqa: E731
         f("cholesky_inplace_dirty").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _err]

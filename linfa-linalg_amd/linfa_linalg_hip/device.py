@@ -169,3 +169,80 @@ class Factor:
             self.close()
         except Exception:  # noqa: BLE001 - interpreter shutdown
             pass
+
+
+class BatchPlan:
+    """A plan for many independent SPD systems of uneven sizes (llh_batch_*).
+
+    Matrix i lives in a packed buffer at a_off[i] (row-major, leading dimension
+    lda[i]); its right-hand sides at b_off[i] (row-major n_i x m_i).  solve()
+    factors every A_i in place and overwrites every B_i with its solution in one
+    hipGraph launch; status[i] is -1 or the failing row."""
+
+    def __init__(self, n, m, prec="f64", small_limit=None, small_waves=None):
+        import numpy as np
+        self._c = context()
+        self.n = np.ascontiguousarray(n, dtype=np.int32)
+        self.m = np.ascontiguousarray(m, dtype=np.int32)
+        self.prec = prec
+        cfg = None
+        if small_limit is not None or small_waves is not None:
+            cfg = _abi.BatchConfig(int(small_limit if small_limit is not None else 128),
+                                   int(small_waves if small_waves is not None else 4))
+        self._ptr = ctypes.c_void_p()
+        err = _abi.LlhError()
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        st = self._c._lib.llh_batch_plan_create(
+            self._c.handle, _abi.PREC[prec], len(self.n), self.n.ctypes.data_as(i32p),
+            self.m.ctypes.data_as(i32p), ctypes.byref(cfg) if cfg is not None else None,
+            ctypes.byref(self._ptr), ctypes.byref(err))
+        _raise(st, err)
+        cnt = len(self.n)
+        self.a_off = np.zeros(cnt, dtype=np.uint64)
+        self.lda = np.zeros(cnt, dtype=np.int32)
+        self.b_off = np.zeros(cnt, dtype=np.uint64)
+        err = _abi.LlhError()
+        _raise(self._c._lib.llh_batch_layout(self._ptr, self.a_off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                             self.lda.ctypes.data_as(i32p),
+                                             self.b_off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                             ctypes.byref(err)), err)
+
+    @property
+    def a_elems(self):
+        return self._c._lib.llh_batch_a_elems(self._ptr)
+
+    @property
+    def b_elems(self):
+        return self._c._lib.llh_batch_b_elems(self._ptr)
+
+    @property
+    def config(self):
+        c = self._c._lib.llh_batch_get_config(self._ptr)
+        return {"small_limit": c.small_limit, "small_waves": c.small_waves}
+
+    def solve(self, a, b, status):
+        err = _abi.LlhError()
+        st = self._c._lib.llh_batch_solve(self._c.handle, self._ptr, ctypes.c_void_p(a.data_ptr()),
+                                          ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(status.data_ptr()),
+                                          ctypes.byref(err))
+        _raise(st, err)
+
+    def autotune(self, a, b, status):
+        err = _abi.LlhError()
+        best = ctypes.c_double(0.0)
+        st = self._c._lib.llh_batch_autotune(self._c.handle, self._ptr, ctypes.c_void_p(a.data_ptr()),
+                                             ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(status.data_ptr()),
+                                             ctypes.byref(best), ctypes.byref(err))
+        _raise(st, err)
+        return best.value
+
+    def close(self):
+        if self._ptr:
+            self._c._lib.llh_batch_plan_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass

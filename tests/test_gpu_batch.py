@@ -1,0 +1,121 @@
+"""Batched solves of uneven SPD systems (llh_batch_*) against the oracle.
+
+Systems taken by the LDS path (n <= 128 f64 / 176 f32) are bit-identical to
+the oracle's solvec; the blocked grouped path is compared within backward-error
+bounds.  A failing system reports its first failing row and leaves the others
+untouched."""
+import numpy as np
+import pytest
+
+import linfa_linalg_hip as llh  # noqa: F401
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(sizes, rhs, dtype, seed):
+    rng = np.random.default_rng(seed)
+    mats, bs = [], []
+    for n, m in zip(sizes, rhs):
+        b = rng.uniform(-1, 1, (n, n))
+        mats.append((b @ b.T / max(n, 1) + np.eye(n)).astype(dtype))
+        bs.append(rng.standard_normal((n, m)).astype(dtype))
+    return mats, bs
+
+
+def pack(plan, mats, bs, dtype):
+    a = np.zeros(plan.a_elems, dtype=dtype)
+    b = np.zeros(plan.b_elems, dtype=dtype)
+    for i, (am, bm) in enumerate(zip(mats, bs)):
+        n, ld = am.shape[0], int(plan.lda[i])
+        o = int(plan.a_off[i])
+        a[o:o + n * ld].reshape(n, ld)[:, :n] = am
+        ob = int(plan.b_off[i])
+        b[ob:ob + bm.size] = bm.ravel()
+    return a, b
+
+
+def unpack(plan, a, b, mats, bs):
+    fac, xs = [], []
+    for i, (am, bm) in enumerate(zip(mats, bs)):
+        n, ld = am.shape[0], int(plan.lda[i])
+        o = int(plan.a_off[i])
+        fac.append(a[o:o + n * ld].reshape(n, ld)[:, :n].copy())
+        ob = int(plan.b_off[i])
+        xs.append(b[ob:ob + bm.size].reshape(bm.shape).copy())
+    return fac, xs
+
+
+def run(plan, mats, bs, dtype):
+    import torch
+    a, b = pack(plan, mats, bs, dtype)
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    st = torch.zeros(len(mats), dtype=torch.int32, device="cuda")
+    plan.solve(ta, tb, st)
+    return unpack(plan, ta.cpu().numpy(), tb.cpu().numpy(), mats, bs) + (st.cpu().numpy(),)
+
+
+SIZES = [1, 2, 5, 17, 32, 33, 64, 65, 100, 128, 129, 176, 177, 200, 256, 300, 333, 512, 513, 700]
+
+
+@pytest.mark.parametrize("dtype,small_limit", [(np.float64, 128), (np.float64, 32), (np.float32, 176)])
+def test_batch_against_oracle(dtype, small_limit):
+    from linfa_linalg_hip import device as dev
+    rng = np.random.default_rng(1)
+    sizes = SIZES + list(rng.integers(1, 400, 40))
+    rhs = [int(x) for x in rng.integers(0, 9, len(sizes))]
+    mats, bs = make_batch(sizes, rhs, dtype, 7)
+    plan = dev.BatchPlan(sizes, rhs, prec="f64" if dtype == np.float64 else "f32", small_limit=small_limit)
+    fac, xs, st = run(plan, mats, bs, dtype)
+    assert np.all(st == -1)
+    eps = np.finfo(dtype).eps
+    for am, bm, f, x in zip(mats, bs, fac, xs):
+        n = am.shape[0]
+        fr, xr = np.array(am), np.array(bm)
+        x_ref = orc.solvec_inplace(fr, xr)
+        if n <= small_limit:
+            assert np.array_equal(f, fr) and np.array_equal(x, x_ref), n
+        else:
+            l = np.tril(f).astype(np.float64)
+            bwd = np.linalg.norm(np.tril(l @ l.T - am)) / np.linalg.norm(np.tril(am))
+            assert bwd <= 20 * eps, (n, bwd)
+            assert np.array_equal(np.triu(f, 1), np.triu(am, 1))  # dirty: upper untouched
+            if bm.size:
+                assert np.abs(x - x_ref).max() <= 1e3 * eps * max(1.0, np.abs(x_ref).max()), n
+
+
+def test_batch_failure_is_isolated_and_replay():
+    from linfa_linalg_hip import device as dev
+    sizes = [50, 300, 90, 400, 20]
+    rhs = [2, 3, 1, 4, 0]
+    mats, bs = make_batch(sizes, rhs, np.float64, 3)
+    mats[1][177, 177] = -1.0   # blocked path fails at row 177
+    mats[2][40, 40] = -1.0     # LDS path fails at row 40
+    plan = dev.BatchPlan(sizes, rhs)
+    for _ in range(2):  # replaying the captured graph gives the same result
+        fac, xs, st = run(plan, mats, bs, np.float64)
+        assert list(st) == [-1, 177, 40, -1, -1]
+        for i in (0, 3):
+            x_ref = orc.solvec(np.array(mats[i]), bs[i])
+            assert np.abs(xs[i] - x_ref).max() <= 1e-12 * max(1, np.abs(x_ref).max())
+
+
+def test_batch_autotune_keeps_inputs():
+    import torch
+    from linfa_linalg_hip import device as dev
+    rng = np.random.default_rng(4)
+    sizes = [int(x) for x in np.exp(rng.uniform(np.log(8), np.log(512), 300))]
+    rhs = [int(x) for x in rng.integers(1, 9, len(sizes))]
+    mats, bs = make_batch(sizes, rhs, np.float64, 5)
+    plan = dev.BatchPlan(sizes, rhs)
+    a, b = pack(plan, mats, bs, np.float64)
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    st = torch.zeros(len(sizes), dtype=torch.int32, device="cuda")
+    best = plan.autotune(ta, tb, st)
+    assert best > 0
+    assert np.array_equal(ta.cpu().numpy(), a) and np.array_equal(tb.cpu().numpy(), b)
+    fac, xs, st2 = run(plan, mats, bs, np.float64)
+    assert np.all(st2 == -1)
+    for am, bm, x in zip(mats, bs, xs):
+        x_ref = orc.solvec(np.array(am), bm)
+        assert np.abs(x - x_ref).max() <= 1e-11 * max(1, np.abs(x_ref).max())
