@@ -56,6 +56,99 @@ def cpu_baseline(n, m, seed):
                       f"A=B*B^T/n+I, best of 2: {t:.2f} s"}
 
 
+def batched_leg(dev, torch, args):
+    """20000 SPD systems, n log-uniform in [8, 512], A_i = B_i B_i^T / n + I with
+    B_i uniform in [-1, 1], 1-8 right-hand sides ~ N(0, 1): one planned,
+    autotuned batch solve (one hipGraph launch), f64; against the same systems
+    solved one plain llh_dev_solvec call at a time (a sample, scaled)."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed + 4)
+    cnt = args.batch
+    sizes = np.floor(np.exp(rng.uniform(np.log(8), np.log(512), cnt))).astype(np.int32)
+    sizes = np.minimum(sizes, 512)
+    rhs = rng.integers(1, 9, cnt).astype(np.int32)
+    plan = dev.BatchPlan(sizes, rhs, prec="f64")
+    a = torch.zeros(plan.a_elems, dtype=torch.float64, device="cuda")
+    b = torch.zeros(plan.b_elems, dtype=torch.float64, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(args.seed + 5)
+    for s in np.unique(sizes):
+        idx = np.nonzero(sizes == s)[0]
+        bm = torch.rand(len(idx), s, s, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+        am = torch.bmm(bm, bm.transpose(1, 2).contiguous()) / s
+        am += torch.eye(int(s), dtype=torch.float64, device="cuda")
+        ld = int(plan.lda[idx[0]])
+        for j, i in enumerate(idx):
+            o = int(plan.a_off[i])
+            a[o:o + s * ld].view(s, ld)[:, :s].copy_(am[j])
+            ob, m = int(plan.b_off[i]), int(rhs[i])
+            b[ob:ob + s * m].normal_(generator=g)
+    status = torch.zeros(cnt, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    log("bench: batched autotune")
+    tuned_ms = plan.autotune(a, b, status)
+    a0, b0 = a.clone(), b.clone()
+    times = []
+    for rep in range(1 + args.steps):
+        a.copy_(a0)
+        b.copy_(b0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.solve(a, b, status)
+        t1 = time.perf_counter()
+        if rep:
+            times.append(t1 - t0)
+    t = float(np.median(times))
+    ok = int((status == -1).sum().item())
+    n64 = sizes.astype(np.float64)
+    flops = float(np.sum(n64 ** 3 / 3 + n64 ** 2 / 2 + n64 / 6 + 2 * n64 ** 2 * rhs))
+    byts = float(np.sum(8 * n64 * (n64 + 1) + 16 * n64 * rhs))
+    roof_t = max(flops / (PEAK_TFLOPS["f64"] * 1e12), byts / (PEAK_HBM_GBS * 1e9))
+    # parity sample against the oracle
+    from oracle import oracle as orc
+    xs = b.cpu().numpy()
+    worst = 0.0
+    for i in rng.choice(cnt, 40, replace=False):
+        s, m = int(sizes[i]), int(rhs[i])
+        o, ld = int(plan.a_off[i]), int(plan.lda[i])
+        am = a0[o:o + s * ld].view(s, ld)[:, :s].cpu().numpy()
+        bm = b0[int(plan.b_off[i]):int(plan.b_off[i]) + s * m].view(s, m).cpu().numpy()
+        xr = orc.solvec(np.array(am), bm)
+        x = xs[int(plan.b_off[i]):int(plan.b_off[i]) + s * m].reshape(s, m)
+        worst = max(worst, float(np.linalg.norm(x - xr) / np.linalg.norm(xr)))
+    # the plain per-matrix path on a sample of the same systems
+    sample = rng.choice(cnt, min(300, cnt), replace=False)
+    mats = []
+    for i in sample:
+        s, m = int(sizes[i]), int(rhs[i])
+        o, ld = int(plan.a_off[i]), int(plan.lda[i])
+        mats.append((a0[o:o + s * ld].view(s, ld)[:, :s].contiguous(),
+                     b0[int(plan.b_off[i]):int(plan.b_off[i]) + s * m].view(s, m).contiguous()))
+    work = [(x.clone(), y.clone()) for x, y in mats]
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for x, y in work:
+        t_a = time.perf_counter()
+        dev.solvec(x, y)
+        lat.append(time.perf_counter() - t_a)
+    t1 = time.perf_counter()
+    plain_rate = len(work) / (t1 - t0)
+    frac = float(np.sum(sizes[sample].astype(np.float64) ** 3)) / float(np.sum(n64 ** 3))
+    plain_rate_scaled = len(work) / ((t1 - t0) / frac * len(work) / cnt) if frac > 0 else None
+    plan.close()
+    return {"workload": f"{cnt} SPD systems, n log-uniform [8,512], 1-8 RHS, f64, one hipGraph launch",
+            "solves_per_s": round(cnt / t, 1), "ms": round(t * 1e3, 3), "gflops": round(flops / t / 1e9, 1),
+            "gbs_algorithmic": round(byts / t / 1e9, 1), "roofline_frac": round(roof_t / t, 4),
+            "roofline_bound": "mfma" if flops / (PEAK_TFLOPS["f64"] * 1e12) > byts / (PEAK_HBM_GBS * 1e9)
+            else "hbm", "config": plan.config if False else None, "autotuned_ms": round(tuned_ms, 3),
+            "ok": ok, "max_rel_err_vs_oracle_sample": worst,
+            "plain_per_matrix": {"solves_per_s_sample": round(plain_rate, 1),
+                                 "solves_per_s_same_mix_est": round(plain_rate_scaled, 1) if plain_rate_scaled
+                                 else None,
+                                 "p50_ms": round(float(np.median(lat)) * 1e3, 3),
+                                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +162,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--batch", type=int, default=20000)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -166,6 +261,14 @@ def main():
         log(f"bench: FP32+IR {ms32:.2f} ms/step, {g32:.1f} GFLOP/s, x{g32 / gflops:.2f} vs f64, {info}")
         del x32
 
+    batched = None
+    if not args.no_batch:
+        del copies
+        torch.cuda.empty_cache()
+        batched = batched_leg(dev, torch, args)
+        log(f"bench: batched {batched['solves_per_s']} solves/s ({batched['ms']} ms), "
+            f"plain {batched['plain_per_matrix']}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log(f"bench: CPU baseline sample n={args.cpu_n}")
@@ -190,6 +293,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp32_ir": mixed,
+        "batched": batched,
         "kernels": kernels,
         "check": check,
     }
