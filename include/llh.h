@@ -270,6 +270,74 @@ llh_status llh_batch_autotune(llh_context* ctx, llh_batch_plan* plan, const void
                               int32_t* status, double* best_ms, llh_error* err);
 void llh_batch_plan_destroy(llh_batch_plan* plan);
 
+/* ---------------------------------------------------------------------------
+ * Solve server: continuous batching of SPD solve requests (no counterpart in
+ * the reference; for callers that serve solve requests as they arrive).
+ *
+ * A worker thread admits submitted requests in arrival order while the HBM
+ * budget and the in-flight limit allow, groups the small new systems admitted
+ * in one step into one batched launch (the exact-order LDS kernel), runs every
+ * other request on one of several HIP streams so their work overlaps, and keeps
+ * each factor resident under its matrix id: a later request with the same id
+ * reuses it and only runs the triangular solves.  Requests arriving while
+ * others run are admitted as slots and memory free up.
+ *
+ * Request buffers are the caller's and must stay valid until the request is
+ * done: a (n x n, f64, lower triangle read) in host memory, or already on the
+ * device when a_on_device != 0; b (n x m) and x (n x m) in host memory,
+ * row-major.  LLH_PREC_F32 requests factor in f32 (f64 diagonal blocks) and
+ * apply `ir_iterations` f64 refinement steps; f64 requests solve directly.
+ * ------------------------------------------------------------------------- */
+typedef struct llh_server llh_server;
+
+typedef struct llh_server_config {
+  uint64_t hbm_budget_bytes; /* device bytes the server may hold (0: 80% of free memory) */
+  int32_t streams;           /* concurrent HIP streams (default 4) */
+  int32_t max_inflight;      /* admitted, unfinished requests (default 64) */
+  int32_t small_limit;       /* new systems with n <= small_limit are batched per step (default 128) */
+  int32_t ir_iterations;     /* refinement steps of f32 requests (default 3) */
+} llh_server_config;
+
+typedef struct llh_request {
+  uint64_t matrix_id;   /* factor cache key; reuse needs the same n */
+  int32_t n;
+  int32_t m;            /* right-hand sides, 1..64 */
+  int32_t precision;    /* llh_precision */
+  int32_t a_on_device;  /* a is a device pointer */
+  const double* a;      /* may be null when the factor is resident */
+  const double* b;
+  double* x;
+} llh_request;
+
+typedef struct llh_result {
+  int32_t status;         /* llh_status of the request */
+  int32_t cache_hit;      /* 1: the resident factor was reused */
+  int32_t batched;        /* 1: ran in a step's batched launch */
+  int32_t precision_used;
+  int64_t failed_row;     /* NotPositiveDefinite row, else -1 */
+  double submit_s, admit_s, done_s; /* steady-clock seconds */
+  double residual;        /* max_j ||b - A x||_inf / (||A||_inf ||x||_inf) for f32 requests, else 0 */
+} llh_result;
+
+typedef struct llh_server_stats {
+  uint64_t submitted, completed, batched, cache_hits, rejected, steps;
+  uint64_t bytes_in_use, bytes_peak, resident_factors;
+} llh_server_stats;
+
+llh_status llh_server_create(llh_context* ctx, const llh_server_config* config, llh_server** out,
+                             llh_error* err);
+/* Non-blocking.  Rejects (LLH_ERR_REJECTED) a request that cannot fit the budget
+ * even alone, or one with bad sizes. */
+llh_status llh_server_submit(llh_server* srv, const llh_request* req, uint64_t* ticket, llh_error* err);
+/* 1 when the request is done (its result is copied to *out and forgotten), else 0. */
+int llh_server_poll(llh_server* srv, uint64_t ticket, llh_result* out);
+llh_status llh_server_wait(llh_server* srv, uint64_t ticket, llh_result* out, llh_error* err);
+void llh_server_get_stats(llh_server* srv, llh_server_stats* out);
+/* Drops resident factors that no request is using. */
+void llh_server_evict_all(llh_server* srv);
+/* Finishes in-flight work, then frees everything. */
+void llh_server_destroy(llh_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

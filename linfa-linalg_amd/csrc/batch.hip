@@ -124,8 +124,7 @@ hipError_t record(llh_batch_plan* p, hipStream_t s) {
         hipSuccess)
       return e;
   for (const auto& st : p->steps) {
-    if ((e = launch_grp_diag<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
-    if ((e = launch_grp_trtri<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
+    if ((e = launch_grp_diag_fast<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
     if ((e = launch_grp_panel<T>(base, items, tw + st.panel_off, st.npanel, s)) != hipSuccess) return e;
     if ((e = launch_grp_fwd<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
     if ((e = launch_grp_syrk<T>(base, items, tw + st.syrk_off, st.nsyrk, s)) != hipSuccess) return e;

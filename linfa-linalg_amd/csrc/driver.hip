@@ -44,21 +44,15 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
         const int b = std::min(kNB, k0 + kb - k);
         T* akk = a + (int64_t)k * lda + k;
         const double db = b;
-        if (diag_f64 && sizeof(T) == 4) {
-          LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * (db + 1.0),
-                   (launch_potrf_exact_single<T, double>(akk, lda, b, info, k, info, s)));
-        } else {
-          LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * (db + 1.0),
-                   (launch_potrf_exact_single<T>(akk, lda, b, info, k, info, s)));
-        }
-        // inv(L11): serves this panel's solve and, later, the triangular solves.
+        // factor + inverse of the diagonal block (the inverse serves this
+        // panel's solve and, later, the triangular solves)
         T* dk = dinv + (int64_t)(k / kNB) * kNB * kNB;
         if (diag_f64 && sizeof(T) == 4) {
-          LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * db,
-                   (launch_trtri_blocks<T, double>(b, akk, lda, false, false, dk, info, s)));
+          LLH_PROF(kDiagFactor, 2.0 * db * db * db / 3.0, es * db * db * 2.0,
+                   (launch_diag_fast<T, double>(akk, lda, b, dk, info, k, info, s)));
         } else {
-          LLH_PROF(kDiagFactor, db * db * db / 3.0, es * db * db,
-                   (launch_trtri_blocks<T>(b, akk, lda, false, false, dk, info, s)));
+          LLH_PROF(kDiagFactor, 2.0 * db * db * db / 3.0, es * db * db * 2.0,
+                   (launch_diag_fast<T>(akk, lda, b, dk, info, k, info, s)));
         }
         const int below = n - k - b;
         if (below <= 0) continue;

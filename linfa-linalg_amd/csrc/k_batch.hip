@@ -31,7 +31,7 @@ __global__ __launch_bounds__(256) void grp_trtri_kernel(const BatchBase<T>* __re
   __shared__ T Xs[kNB * (kNB + 1)];
   __shared__ T Ts[kNB * (kNB + 1)];
   const T* a = bb.a + it.a_off + (int64_t)w.k0 * it.lda + w.k0;
-  trtri64_block<T, T>(min(kNB, it.n - w.k0), a, it.lda, false, false,
+  trtri64_block<T, T, T>(min(kNB, it.n - w.k0), a, it.lda, false, false,
                       bb.linv + it.linv_off + (int64_t)(w.k0 / kNB) * kNB * kNB, Ms, Xs, Ts);
 }
 

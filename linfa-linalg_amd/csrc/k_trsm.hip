@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void trtri64_kernel(int n, const TS* __restric
   __shared__ T Xs[kNB * (kNB + 1)];
   __shared__ T Ts[kNB * (kNB + 1)];
   const int o = blockIdx.x * kNB;
-  trtri64_block<TS, T>(min(kNB, n - o), A + (int64_t)o * lda + o, lda, trans, upper,
+  trtri64_block<TS, TS, T>(min(kNB, n - o), A + (int64_t)o * lda + o, lda, trans, upper,
                        out + (int64_t)blockIdx.x * kNB * kNB, Ms, Xs, Ts);
 }
 

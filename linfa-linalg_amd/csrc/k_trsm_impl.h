@@ -17,9 +17,12 @@ namespace llh {
 // block is upper, with exact zeros outside the triangle.
 // A points at the block's origin, b <= 64 is its order; dst receives the 64 x 64
 // row-major inverse.  Ms, Xs, Ts are LDS arrays of 64 x 65 elements.
-template <typename TS, typename T>
-__device__ void trtri64_block(int b, const TS* __restrict__ A, int64_t lda, bool trans, bool upper,
-                              TS* __restrict__ dst, T* Ms, T* Xs, T* Ts) {
+// TA: element type of A (global or LDS, read through a generic pointer); TD:
+// element type of dst; T: arithmetic type.  A may alias Ms when trans == upper
+// (each element is then read and rewritten by the same thread).
+template <typename TA, typename TD, typename T>
+__device__ void trtri64_block(int b, const TA* A, int64_t lda, bool trans, bool upper, TD* __restrict__ dst,
+                              T* Ms, T* Xs, T* Ts) {
   constexpr int S = kNB + 1;
   const int tid = threadIdx.x;
   const int o = 0;
@@ -27,12 +30,12 @@ __device__ void trtri64_block(int b, const TS* __restrict__ A, int64_t lda, bool
   // with op(A) = A^T when trans, M[r][c] is A[o+c][o+r] exactly when upper != trans.
   const bool flip = (upper != trans);
   {
-    TS v[16];  // all 16 loads of a thread in flight before any store
+    TA v[16];  // all 16 loads of a thread in flight before any store
 #pragma unroll
     for (int g = 0; g < 16; ++g) {
       const int e = tid + 256 * g, i = e / kNB, j = e % kNB;  // coalesced over j in A's storage
       const int r = flip ? j : i, c = flip ? i : j;
-      v[g] = (r < b && c < b && c <= r) ? A[(int64_t)(o + i) * lda + (o + j)] : TS(0);
+      v[g] = (r < b && c < b && c <= r) ? A[(int64_t)(o + i) * lda + (o + j)] : TA(0);
     }
 #pragma unroll
     for (int g = 0; g < 16; ++g) {
@@ -84,7 +87,7 @@ __device__ void trtri64_block(int b, const TS* __restrict__ A, int64_t lda, bool
   }
   for (int e = tid; e < kNB * kNB; e += 256) {
     const int r = e / kNB, c = e % kNB;
-    dst[e] = TS(upper ? Xs[c * S + r] : Xs[r * S + c]);
+    dst[e] = TD(upper ? Xs[c * S + r] : Xs[r * S + c]);
   }
 }
 

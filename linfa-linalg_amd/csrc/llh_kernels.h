@@ -97,6 +97,17 @@ hipError_t launch_grp_bwd(const BatchBase<T>* base, const BatchItem* items, cons
                           hipStream_t s);
 hipError_t launch_fill_status(int32_t* status, int64_t count, hipStream_t s);
 
+// k_diag.hip: fast factor of one b x b (b <= 64) diagonal block in place, and
+// its inverse into inv (64 x 64 row-major); a failure at block row f sets
+// *info = min(*info, info_base + f).
+template <typename TS, typename T = TS>
+hipError_t launch_diag_fast(TS* a, int64_t lda, int b, TS* inv, int32_t* info, int64_t info_base,
+                            const int32_t* gate, hipStream_t s);
+// k_diag.hip: grouped form over step work (factor + inverse of each block).
+template <typename T>
+hipError_t launch_grp_diag_fast(const BatchBase<T>* base, const BatchItem* items, const StepWork* work,
+                                int count, hipStream_t s);
+
 // k_gemm.hip: C = beta*C + alpha*op(A)*op(B), optionally lower-triangle only.
 template <typename T>
 hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha, const T* A, int64_t lda,

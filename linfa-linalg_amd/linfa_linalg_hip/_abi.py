@@ -46,6 +46,30 @@ class BatchConfig(ctypes.Structure):
     _fields_ = [("small_limit", ctypes.c_int32), ("small_waves", ctypes.c_int32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("streams", ctypes.c_int32),
+                ("max_inflight", ctypes.c_int32), ("small_limit", ctypes.c_int32),
+                ("ir_iterations", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("matrix_id", ctypes.c_uint64), ("n", ctypes.c_int32), ("m", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("a_on_device", ctypes.c_int32), ("a", ctypes.c_void_p),
+                ("b", ctypes.c_void_p), ("x", ctypes.c_void_p)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("cache_hit", ctypes.c_int32), ("batched", ctypes.c_int32),
+                ("precision_used", ctypes.c_int32), ("failed_row", ctypes.c_int64),
+                ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("residual", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "completed", "batched", "cache_hits", "rejected",
+                                               "steps", "bytes_in_use", "bytes_peak", "resident_factors")]
+
+
 PREC = {"f64": 0, "f32": 1}
 FALLBACK_F64 = 1
 
@@ -88,6 +112,16 @@ def _declare(lib):
     lib.llh_batch_autotune.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_double), _err]
     lib.llh_batch_plan_destroy.argtypes = [_vp]
     lib.llh_batch_plan_destroy.restype = None
+    lib.llh_server_create.argtypes = [_vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp), _err]
+    lib.llh_server_submit.argtypes = [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64), _err]
+    lib.llh_server_poll.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(Result)]
+    lib.llh_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(Result), _err]
+    lib.llh_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
+    lib.llh_server_get_stats.restype = None
+    lib.llh_server_evict_all.argtypes = [_vp]
+    lib.llh_server_evict_all.restype = None
+    lib.llh_server_destroy.argtypes = [_vp]
+    lib.llh_server_destroy.restype = None
     for s in ("d", "s"):
         f = lambda n: getattr(lib, f"llh_{n}_{s}")  # noqa: E731
         f("cholesky_inplace_dirty").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _err]

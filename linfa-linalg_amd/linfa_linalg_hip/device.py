@@ -246,3 +246,90 @@ class BatchPlan:
             self.close()
         except Exception:  # noqa: BLE001 - interpreter shutdown
             pass
+
+
+class Server:
+    """Continuous-batching solve server (llh_server_*).
+
+    submit() takes a host (numpy) or device (torch) f64 matrix a (n x n,
+    C-contiguous; None when the factor under matrix_id is resident), host b
+    (n x m) and host x (n x m, written); the arrays are kept alive until the
+    request is collected with wait() or poll()."""
+
+    def __init__(self, budget_bytes=0, streams=4, max_inflight=64, small_limit=128, ir_iterations=3):
+        from . import default_context
+        self._c = default_context()
+        self._ptr = ctypes.c_void_p()
+        cfg = _abi.ServerConfig(int(budget_bytes), int(streams), int(max_inflight), int(small_limit),
+                                int(ir_iterations))
+        err = _abi.LlhError()
+        _raise(self._c._lib.llh_server_create(self._c.handle, ctypes.byref(cfg), ctypes.byref(self._ptr),
+                                              ctypes.byref(err)), err)
+        self._keep = {}
+
+    def submit(self, matrix_id, a, b, x, precision="f64"):
+        import numpy as np
+        n, m = b.shape
+        on_dev = a is not None and not isinstance(a, np.ndarray)
+        if a is not None:
+            if on_dev:
+                _check(a)
+                a_ptr = a.data_ptr()
+            else:
+                assert a.dtype == np.float64 and a.flags.c_contiguous and a.shape == (n, n)
+                a_ptr = a.ctypes.data
+        else:
+            a_ptr = None
+        assert b.dtype == np.float64 and b.flags.c_contiguous
+        assert x.dtype == np.float64 and x.flags.c_contiguous and x.shape == (n, m) and x.flags.writeable
+        req = _abi.Request(int(matrix_id), n, m, _abi.PREC[precision], int(on_dev), a_ptr, b.ctypes.data,
+                           x.ctypes.data)
+        t = ctypes.c_uint64(0)
+        err = _abi.LlhError()
+        _raise(self._c._lib.llh_server_submit(self._ptr, ctypes.byref(req), ctypes.byref(t), ctypes.byref(err)), err)
+        self._keep[t.value] = (a, b, x)
+        return t.value
+
+    @staticmethod
+    def _res(r):
+        return {"status": r.status, "cache_hit": bool(r.cache_hit), "batched": bool(r.batched),
+                "precision_used": {v: k for k, v in _abi.PREC.items()}.get(r.precision_used),
+                "failed_row": r.failed_row, "latency_s": r.done_s - r.submit_s,
+                "queue_s": r.admit_s - r.submit_s if r.admit_s > 0 else None, "residual": r.residual,
+                "done_s": r.done_s, "submit_s": r.submit_s}
+
+    def poll(self, ticket):
+        r = _abi.Result()
+        if self._c._lib.llh_server_poll(self._ptr, ticket, ctypes.byref(r)):
+            self._keep.pop(ticket, None)
+            return self._res(r)
+        return None
+
+    def wait(self, ticket):
+        r = _abi.Result()
+        err = _abi.LlhError()
+        st = self._c._lib.llh_server_wait(self._ptr, ticket, ctypes.byref(r), ctypes.byref(err))
+        self._keep.pop(ticket, None)
+        out = self._res(r)
+        out["error"] = None if st == 0 else _abi.status_name(st)
+        return out
+
+    def stats(self):
+        st = _abi.ServerStats()
+        self._c._lib.llh_server_get_stats(self._ptr, ctypes.byref(st))
+        return {k: getattr(st, k) for k, _ in _abi.ServerStats._fields_}
+
+    def evict_all(self):
+        self._c._lib.llh_server_evict_all(self._ptr)
+
+    def close(self):
+        if self._ptr:
+            self._c._lib.llh_server_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
