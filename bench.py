@@ -149,6 +149,86 @@ def batched_leg(dev, torch, args):
                                  "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}}
 
 
+def serving_leg(dev, torch, args):
+    """Poisson arrivals of solve requests at several offered rates: n log-uniform
+    in [16, 4096], A = B B^T / n + I with B uniform in [-1, 1] (pre-staged in HBM,
+    so the request stream is not PCIe-bound), 1-32 right-hand sides, 30% of the
+    requests reuse an earlier matrix id, f64 and FP32+IR half and half.  Latency
+    is submit -> solution in the caller's host buffer."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed + 7)
+    npool = 64
+    pool_n = np.floor(np.exp(rng.uniform(np.log(16), np.log(4096), npool))).astype(int)
+    g = torch.Generator(device="cuda").manual_seed(args.seed + 8)
+    pool = []
+    for n in pool_n:
+        bm = torch.rand(int(n), int(n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+        a = bm @ bm.T / int(n)
+        a.diagonal().add_(1.0)
+        pool.append(a)
+    torch.cuda.synchronize()
+    out = {"workload": "Poisson solve requests, n log-uniform [16,4096] (A pre-staged in HBM), 1-32 RHS, "
+                       "30% id reuse, 50/50 f64 and FP32+IR", "rates": []}
+    for rate, dur in ((1000, 2.0), (10000, 0.6), (50000, 0.2)):
+        log(f"bench: serving at {rate}/s for {dur} s")
+        srv = dev.Server()
+        cnt = int(rate * dur)
+        gaps = rng.exponential(1.0 / rate, cnt)
+        reqs = []
+        seen = []
+        next_id = 1
+        for i in range(cnt):
+            if seen and rng.random() < 0.3:
+                mid, pi = seen[int(rng.integers(0, len(seen)))]
+            else:
+                mid, pi = next_id, int(rng.integers(0, npool))
+                next_id += 1
+                seen.append((mid, pi))
+            n, m = int(pool_n[pi]), int(rng.integers(1, 33))
+            prec = "f32" if rng.random() < 0.5 else "f64"
+            reqs.append((mid, pi, n, m, prec, rng.standard_normal((n, m)), np.zeros((n, m))))
+        tickets = []
+        t0 = time.perf_counter()
+        tnext = t0
+        for (mid, pi, n, m, prec, b, x), gap in zip(reqs, gaps):
+            tnext += gap
+            while time.perf_counter() < tnext:
+                pass
+            tickets.append(srv.submit(mid, pool[pi], b, x, precision=prec))
+        t_sub = time.perf_counter() - t0
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+        srv.close()
+        lat = np.array([r["latency_s"] for r in res])
+        okr = [r for r in res if r["status"] == 0]
+        span = max(r["done_s"] for r in res) - min(r["submit_s"] for r in res)
+        out["rates"].append({"offered_per_s": rate, "submitted": cnt, "achieved_offered_per_s": round(cnt / t_sub, 1),
+                             "completed_ok": len(okr), "solves_per_s": round(len(okr) / span, 1),
+                             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                             "cache_hits": st["cache_hits"], "batched": st["batched"], "steps": st["steps"],
+                             "peak_gb": round(st["bytes_peak"] / 1e9, 2)})
+        log(f"bench: serving {out['rates'][-1]}")
+        del reqs
+    # the plain path: the same kind of requests one at a time (llh_dev_solvec_d on a fresh copy)
+    lat = []
+    t0 = time.perf_counter()
+    for i in range(100):
+        pi = int(rng.integers(0, npool))
+        n, m = int(pool_n[pi]), int(rng.integers(1, 33))
+        b = rng.standard_normal((n, m))
+        ta = pool[pi].clone()
+        t_a = time.perf_counter()
+        tb = torch.from_numpy(b).cuda()
+        dev.solvec(ta, tb)
+        tb.cpu()
+        lat.append(time.perf_counter() - t_a)
+    out["plain_sequential"] = {"solves_per_s": round(100 / sum(lat), 1),
+                               "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                               "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +244,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--batch", type=int, default=20000)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -213,11 +294,13 @@ def main():
                          "avg_launch_us": round(tot_ms / cnt * 1e3, 2),
                          "tflops": round(fl / (tot_ms / 1e3) / 1e12, 2) if tot_ms > 0 else None,
                          "gbs": round(by / (tot_ms / 1e3) / 1e9, 1) if tot_ms > 0 else None}
+    log("bench: kernels " + json.dumps(kernels))
     cnt, tot_ms, fl, by = stats["trailing_syrk"]
     achieved = fl / (tot_ms / 1e3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_TFLOPS["f64"], 4), "traffic": None,
-                "kernel": "gemm_kernel<double,128,128,2,2,TA=0,TB=1,TRI=1> (trailing SYRK, rank 256)",
+                "kernel": "gemm_nt_kernel<double,256,128,4,2,TRI=1> (trailing SYRK, rank 256; 128x128 tiles once the "
+                          "trailing matrix is below 4096^2)",
                 "avg_launch_us": round(tot_ms / cnt * 1e3, 2), "launches_per_step": cnt // args.steps,
                 "algorithmic_tflop_per_step": round(fl / args.steps / 1e12, 4)}
 
@@ -269,6 +352,10 @@ def main():
         log(f"bench: batched {batched['solves_per_s']} solves/s ({batched['ms']} ms), "
             f"plain {batched['plain_per_matrix']}")
 
+    serving = None
+    if not args.no_serve:
+        serving = serving_leg(dev, torch, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         log(f"bench: CPU baseline sample n={args.cpu_n}")
@@ -294,6 +381,7 @@ def main():
         "cpu_baseline": cpu,
         "fp32_ir": mixed,
         "batched": batched,
+        "serving": serving,
         "kernels": kernels,
         "check": check,
     }

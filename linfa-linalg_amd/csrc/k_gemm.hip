@@ -163,10 +163,51 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_kernel(int M, int N, int K, 
 //
 // Tile order: WG ids are dealt round-robin to the 8 XCDs, so the id is first
 // turned into (xcd, slot) and each XCD receives a contiguous run of tiles; tiles
-// are ordered in groups of GROUP_M tile rows, column by column within a group, so
-// the WGs resident on one XCD share a few A and B row panels in its L2.  For the
-// square triangular case only the lower tiles are enumerated.
+// are ordered in groups of kGroupM tile rows, column by column within a group, so
+// the WGs resident on one XCD share a few A and B row panels in its L2.
+//
+// For the square triangular case (M == N) only the tiles that meet the lower
+// triangle are enumerated: with BM = R * BN, tile row tm needs the columns
+// tn < R * (tm + 1).  Enumerating the full rectangle instead would hand the XCDs
+// that own the top tile rows mostly empty tiles and the last ones only full tiles.
 constexpr int kGroupM = 8;
+
+__host__ __device__ inline int tri_group_tiles(int g0, int gm, int tiles_n, int R) {
+  const int cg = min(tiles_n, R * (g0 + gm));
+  int cnt = R * g0 * gm;  // columns left of the group's diagonal band: all gm rows
+  for (int c = R * g0; c < cg; ++c) cnt += g0 + gm - c / R;
+  return cnt;
+}
+
+__host__ __device__ inline int tri_total(int tiles_m, int tiles_n, int R) {
+  int total = 0;
+  for (int g0 = 0; g0 < tiles_m; g0 += kGroupM) total += tri_group_tiles(g0, min(kGroupM, tiles_m - g0), tiles_n, R);
+  return total;
+}
+
+__device__ inline void tri_tile(int lin, int tiles_m, int tiles_n, int R, int& tm, int& tn) {
+  int rem = lin, g0 = 0;
+  for (;;) {
+    const int cnt = tri_group_tiles(g0, min(kGroupM, tiles_m - g0), tiles_n, R);
+    if (rem < cnt) break;
+    rem -= cnt;
+    g0 += kGroupM;
+  }
+  const int gm = min(kGroupM, tiles_m - g0);
+  if (rem < R * g0 * gm) {
+    tn = rem / gm;
+    tm = g0 + rem % gm;
+    return;
+  }
+  rem -= R * g0 * gm;
+  int c = R * g0;  // column c of the band starts at tile row c / R
+  while (rem >= g0 + gm - c / R) {
+    rem -= g0 + gm - c / R;
+    ++c;
+  }
+  tn = c;
+  tm = c / R + rem;
+}
 
 template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool VEC>
 __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_kernel(int M, int N, int K, T alpha,
@@ -175,6 +216,7 @@ __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_ker
                                                                  T* __restrict__ C, int64_t ldc,
                                                                  const int32_t* __restrict__ gate, int tiles_m,
                                                                  int tiles_n, int total) {
+  static_assert(BM % BN == 0, "tile rows are a multiple of tile columns");
   if (gate != nullptr && *gate != kNoFail) return;
 
   // XCD-aware linear tile index (the launch has exactly 8 * per WGs)
@@ -184,30 +226,8 @@ __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_ker
   const int lin = (id % nx) * per + id / nx;
   if (lin >= total) return;
   int tm, tn;
-  if (TRI && tiles_m == tiles_n) {
-    // lower tiles only; group g covers tile rows [g0, g0 + gm)
-    int rem = lin, g0 = 0;
-    for (;;) {
-      const int gm = min(kGroupM, tiles_m - g0);
-      const int cnt = g0 * gm + gm * (gm + 1) / 2;
-      if (rem < cnt) break;
-      rem -= cnt;
-      g0 += kGroupM;
-    }
-    const int gm = min(kGroupM, tiles_m - g0);
-    if (rem < g0 * gm) {
-      tn = rem / gm;
-      tm = g0 + rem % gm;
-    } else {
-      rem -= g0 * gm;  // triangle of the group: column g0 + j has gm - j tiles
-      int j = 0;
-      while (rem >= gm - j) {
-        rem -= gm - j;
-        ++j;
-      }
-      tn = g0 + j;
-      tm = g0 + j + rem;
-    }
+  if (TRI && M == N) {
+    tri_tile(lin, tiles_m, tiles_n, BM / BN, tm, tn);
   } else {
     const int per_group = kGroupM * tiles_n;
     const int g = lin / per_group;
@@ -227,7 +247,7 @@ template <typename T, int BM, int BN, int WM, int WN, bool TRI>
 hipError_t launch_nt(int M, int N, int K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
                      T* C, int64_t ldc, const int32_t* gate, hipStream_t s) {
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
-  const int total = (TRI && tiles_m == tiles_n) ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
+  const int total = (TRI && M == N) ? tri_total(tiles_m, tiles_n, BM / BN) : tiles_m * tiles_n;
   const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
                    ((lda * sizeof(T)) % 16 == 0) && ((ldb * sizeof(T)) % 16 == 0);
   // The kernel deals WG ids to XCDs as lin = (id % 8) * per + id / 8, which is a

@@ -46,6 +46,7 @@ struct Entry {
   bool failed = false;
   int64_t failed_row = -1;
   bool dinv_ready = false;
+  bool in_step = false;  // factored by the batched step that has not been launched yet
 };
 
 // Pinned host staging, recycled by power-of-two size class.
@@ -73,6 +74,15 @@ struct PinnedPool {
   }
 };
 
+// Host staging of one batched step's descriptors: it must outlive the step's
+// asynchronous upload, so it is released with the step's event.
+struct StepStage {
+  hipEvent_t done = nullptr;
+  void* host = nullptr;
+  size_t cls = 0;
+  int remaining = 0;
+};
+
 struct Job {
   uint64_t ticket = 0;
   llh_request req{};
@@ -87,6 +97,7 @@ struct Job {
   size_t host_cls = 0;
   hipEvent_t done = nullptr;
   size_t reserved = 0;
+  StepStage* step = nullptr;  // batched jobs share their step's staging and event
 };
 
 struct StreamState {
@@ -367,8 +378,14 @@ bool admit(llh_server* s, Job& j) {
     fail_job(s, j, LLH_ERR_INVALID_ARGUMENT, "matrix not resident and no matrix given");
     return true;
   }
-  j.stream = s->next_stream;
-  s->next_stream = (s->next_stream + 1) % (int)s->streams.size();
+  // Small new systems run in the step's batched launch on stream 0, so their
+  // stream-ordered allocations are made there; everything else round-robins.
+  if (small_new || s->streams.size() == 1) {
+    j.stream = 0;
+  } else {
+    j.stream = 1 + s->next_stream % ((int)s->streams.size() - 1);
+    s->next_stream = (s->next_stream + 1) % ((int)s->streams.size() - 1);
+  }
   hipStream_t st = s->streams[j.stream].s;
   size_t hb = (size_t)r.n * r.m * 8 + (2 * r.m + 2) * 8;
   j.host = s->pinned.get(hb, &j.host_cls);
@@ -424,9 +441,9 @@ bool admit(llh_server* s, Job& j) {
 }
 
 void launch_small_step(llh_server* s, std::vector<Job*>& batch) {
-  // One stream, one launch of the exact-order kernel over every new small system.
-  const int si = s->next_stream;
-  s->next_stream = (s->next_stream + 1) % (int)s->streams.size();
+  // One launch of the exact-order kernel over every new small system, on the
+  // stream their buffers were allocated on.
+  const int si = 0;
   StreamState& st = s->streams[si];
   std::vector<SmallDesc<double>> descs;
   int nmax = 0;
@@ -461,6 +478,7 @@ void launch_small_step(llh_server* s, std::vector<Job*>& batch) {
     dev_free(st.s, dd);
   }
   hipEvent_t ev = get_event(s);
+  auto* stage = new StepStage{ev, hd, cls, (int)batch.size()};
   for (Job* j : batch) {
     Entry* e = j->entry;
     const size_t nm = (size_t)e->n * j->req.m;
@@ -470,12 +488,14 @@ void launch_small_step(llh_server* s, std::vector<Job*>& batch) {
     (void)hipMemcpyAsync(reinterpret_cast<int32_t*>(hx + nm + 2 * j->req.m), e->d_info, 4, hipMemcpyDeviceToHost,
                          st.s);
     (void)hipEventRecord(e->ready, st.s);
+    e->in_step = false;
     j->batched = true;
     j->done = ev;
+    j->step = stage;
   }
   (void)hipEventRecord(ev, st.s);
-  if (hd) s->pinned.put(hd, cls);
-  // the shared event is recycled after the last job of the step finishes
+  // the staging block and the shared event are recycled after the step's last
+  // job finishes (finish_step)
   std::lock_guard<std::mutex> g(s->mu);
   s->stats.batched += batch.size();
   s->stats.steps++;
@@ -508,14 +528,13 @@ void worker_loop(llh_server* s) {
     // completions
     for (auto it = s->jobs.begin(); it != s->jobs.end();) {
       if (hipEventQuery(it->done) == hipSuccess) {
-        hipEvent_t ev = it->done;
-        const bool shared = it->batched;
+        StepStage* stage = it->step;
         finish(s, *it);
         it = s->jobs.erase(it);
-        if (shared) {
-          bool still = false;
-          for (auto& o : s->jobs) still |= (o.done == ev);
-          if (!still) s->event_pool.push_back(ev);
+        if (stage && --stage->remaining == 0) {
+          if (stage->host) s->pinned.put(stage->host, stage->cls);
+          s->event_pool.push_back(stage->done);
+          delete stage;
         }
         progressed = true;
       } else {
@@ -539,8 +558,15 @@ void worker_loop(llh_server* s) {
       s->jobs.push_back(std::move(j));
       Job& q = s->jobs.back();
       if (!q.hit && q.entry->n <= s->cfg.small_limit) {
+        q.entry->in_step = true;
         small.push_back(&q);
         continue;
+      }
+      // a hit on a system the pending step factors: launch that step first, so
+      // the entry's ready event is recorded before this job waits on it
+      if (q.entry->in_step) {
+        launch_small_step(s, small);
+        small.clear();
       }
       StreamState& st = s->streams[q.stream];
       hipError_t x = hipSuccess;

@@ -107,3 +107,37 @@ def test_failure_and_rejection_are_isolated():
         big = np.zeros((3000, 3000))
         s.submit(4, big, np.ones((3000, 1)), np.zeros((3000, 1)))  # 72 MB+ > 64 MB budget
     s.close()
+
+
+def test_small_burst_with_repeated_ids():
+    """Hundreds of small requests submitted at once, ids repeating inside the
+    burst, host and device matrices mixed: hits on a system that the pending
+    batched step factors wait for that step, and every step's descriptors stay
+    intact until its upload has run."""
+    import torch
+    from linfa_linalg_hip import device as dev
+    rng = np.random.default_rng(5)
+    sizes = [8, 33, 64, 100, 128]
+    mats = {i: spd(sizes[i % len(sizes)], 300 + i) for i in range(20)}
+    dmats = {i: torch.from_numpy(a).cuda() for i, a in mats.items() if i % 3 == 0}
+    s = dev.Server(streams=4)
+    reqs = []
+    for k in range(400):
+        mid = int(rng.integers(0, 20))
+        n = mats[mid].shape[0]
+        m = int(rng.integers(1, 17))
+        b = rng.standard_normal((n, m))
+        x = np.zeros((n, m))
+        src = dmats.get(mid, mats[mid])
+        reqs.append((mid, b, x, s.submit(mid, src, b, x)))
+    res = [s.wait(t) for (_, _, _, t) in reqs]
+    stats = s.stats()
+    s.close()
+    assert stats["cache_hits"] > 0 and stats["batched"] > 0
+    for (mid, b, x, _), r in zip(reqs, res):
+        assert r["status"] == 0, (mid, r)
+        x_ref = orc.solvec(np.array(mats[mid]), b)
+        err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+        assert err <= 1e-12, (mid, err, r)
+        if r["batched"]:
+            assert np.array_equal(x, x_ref), mid  # exact-order path for new small systems
