@@ -229,6 +229,118 @@ def serving_leg(dev, torch, args):
     return out
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trailing_syrk_f64_n32768_traffic.json")
+
+
+def _is_trailing_syrk(name):
+    # the square lower-triangle instantiation (TRI, SQ) of the NT MFMA kernel
+    return name.startswith("void llh::(anonymous namespace)::gemm_nt_kernel<double,") and ", true, true, " in name
+
+
+def rocprof_summary(trace_dir, fetch_dir, write_dir, n, out_md, out_json, command):
+    """Summarises one `rocprofv3 --kernel-trace --stats` run and two `--pmc`
+    runs (FETCH_SIZE, WRITE_SIZE) of this benchmark: per-kernel average
+    durations, and HBM bytes per trailing-SYRK launch scaled by what the same
+    counters read for the n x n f64 clone of A (a plain copy of known size)."""
+    import csv
+    import glob
+
+    def one(pattern):
+        hits = glob.glob(os.path.join(pattern))
+        if not hits:
+            raise SystemExit(f"no file matches {pattern}")
+        return hits[0]
+
+    stats = list(csv.DictReader(open(one(os.path.join(trace_dir, "*kernel_stats.csv")))))
+    # The TRI+SQ instantiation also runs the square inner-panel updates of the
+    # last outer block (below == cols > 64); replay the factorization's launch
+    # order to label each of its launches.
+    pattern = []
+    k0 = 0
+    while k0 < n:
+        kb = min(256, n - k0)
+        for k in range(k0, k0 + kb, 64):
+            b = min(64, k0 + kb - k)
+            below, cols = n - k - b, k0 + kb - (k + b)
+            if below > 0 and cols > 64 and below == cols:
+                pattern.append(False)
+        if n - k0 - kb > 0:
+            pattern.append(True)
+        k0 += kb
+    trace = sorted((r for r in csv.DictReader(open(one(os.path.join(trace_dir, "*kernel_trace.csv"))))
+                    if _is_trailing_syrk(r["Kernel_Name"])), key=lambda r: int(r["Start_Timestamp"]))
+    durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for i, r in enumerate(trace)
+            if pattern[i % len(pattern)]]
+    syrk_calls, syrk_ns = len(durs), float(sum(durs))
+    counters = {}
+    for ctr, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = [r for r in csv.DictReader(open(one(os.path.join(d, "*counter_collection.csv"))))
+                if r["Counter_Name"] == ctr]
+        copy = max(float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"])
+        known_kib = n * n * 8 / 1024.0
+        factor = known_kib / copy
+        sq = sorted((r for r in rows if _is_trailing_syrk(r["Kernel_Name"])), key=lambda r: int(r["Dispatch_Id"]))
+        vals = [float(r["Counter_Value"]) for i, r in enumerate(sq) if pattern[i % len(pattern)]]
+        counters[ctr] = {"copy_kib": copy, "known_kib": known_kib, "factor": factor, "launches": len(vals),
+                         "avg_kib": sum(vals) / len(vals), "bytes_per_launch": sum(vals) / len(vals) * 1024 * factor}
+    traffic = counters["FETCH_SIZE"]["bytes_per_launch"] + counters["WRITE_SIZE"]["bytes_per_launch"]
+    # algorithmic bytes of the trailing SYRKs of one factorization (outer block 256)
+    alg = []
+    k0 = 0
+    while k0 < n:
+        kb = min(256, n - k0)
+        m2 = n - k0 - kb
+        if m2 > 0:
+            alg.append(8.0 * (m2 * (m2 + 1) + m2 * kb))
+        k0 += kb
+    alg_avg = sum(alg) / len(alg)
+    res = {"n": n, "kernel_class": "trailing SYRK (gemm_nt_kernel<double,...,TRI,SQ>)",
+           "avg_launch_us": syrk_ns / syrk_calls / 1e3, "launches": syrk_calls,
+           "fetch": counters["FETCH_SIZE"], "write": counters["WRITE_SIZE"],
+           "traffic_bytes_per_launch": traffic, "algorithmic_bytes_per_launch": alg_avg,
+           "traffic_over_algorithmic": traffic / alg_avg, "command": command}
+    with open(out_json, "w") as f:
+        json.dump(res, f, indent=1)
+    lines = [f"# rocprofv3 summary: f64 SPD solve, n={n}, 16 RHS (bench.py)", "",
+             "Commands (run from the repository root on one MI355X):", "", "```", command, "```", "",
+             "## Kernel durations (`--kernel-trace --stats`)", "",
+             "| kernel | calls | avg us | total ms | % |", "|---|---:|---:|---:|---:|"]
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:14]:
+        name = r["Name"]
+        short = name.replace("void ", "").replace("(anonymous namespace)::", "").split("(")[0]
+        if len(short) > 90:
+            short = short[:87] + "..."
+        lines.append(f"| `{short}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.1f} | {float(r['Percentage']):.1f} |")
+    lines += ["", f"Trailing SYRK (the TRI+SQ instantiation of `gemm_nt_kernel`, 256x128 tiles while the trailing "
+                  f"matrix is at least 4096^2, 128x128 below; the square inner-panel updates of the last "
+                  f"256-column block, which share the instantiation, left out): {syrk_calls} launches, "
+                  f"average {syrk_ns / syrk_calls / 1e3:.1f} us per launch.", "",
+              "## HBM traffic per trailing-SYRK launch (`--pmc FETCH_SIZE`, then `--pmc WRITE_SIZE`)", "",
+              "| counter | copy of A reads (KiB) | known (KiB) | factor | avg per launch (KiB) | bytes per launch |",
+              "|---|---:|---:|---:|---:|---:|"]
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        c = counters[ctr]
+        lines.append(f"| {ctr} | {c['copy_kib']:.0f} | {c['known_kib']:.0f} | {c['factor']:.3f} | "
+                     f"{c['avg_kib']:.0f} | {c['bytes_per_launch'] / 1e9:.3f} GB |")
+    lines += ["", f"Traffic {traffic / 1e9:.3f} GB per launch against {alg_avg / 1e9:.3f} GB algorithmic "
+                  f"(read and write the lower triangle of C once, read the panel once): "
+                  f"{traffic / alg_avg:.2f}x.", ""]
+    with open(out_md, "w") as f:
+        f.write("\n".join(lines))
+    print(json.dumps(res))
+
+
+def load_traffic(n, m):
+    """Per-launch HBM bytes of the trailing SYRK from the committed PMC summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t if t.get("n") == n and m == 16 else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,7 +357,17 @@ def main():
     ap.add_argument("--batch", type=int, default=20000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--rocprof-summary", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
+                    help="summarise rocprofv3 runs of this benchmark into profiles/ and exit")
+    ap.add_argument("--rocprof-command", default="")
     args = ap.parse_args()
+
+    if args.rocprof_summary:
+        rocprof_summary(*args.rocprof_summary, args.n,
+                        os.path.join(ROOT, "profiles", f"trailing_syrk_f64_n{args.n}.md"),
+                        os.path.join(ROOT, "profiles", f"trailing_syrk_f64_n{args.n}_traffic.json"),
+                        args.rocprof_command)
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -297,8 +419,14 @@ def main():
     log("bench: kernels " + json.dumps(kernels))
     cnt, tot_ms, fl, by = stats["trailing_syrk"]
     achieved = fl / (tot_ms / 1e3) / 1e12
+    tr = load_traffic(n, m)
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_TFLOPS["f64"], 4), "traffic": None,
+                "frac": round(achieved / PEAK_TFLOPS["f64"], 4),
+                "traffic": round(tr["traffic_bytes_per_launch"]) if tr else None,
+                "traffic_note": ("HBM bytes per launch (rocprofv3 FETCH_SIZE x%.2f + WRITE_SIZE x%.2f, factors from "
+                                 "a copy of A), %.2fx the algorithmic bytes; profiles/trailing_syrk_f64_n%d.md"
+                                 % (tr["fetch"]["factor"], tr["write"]["factor"], tr["traffic_over_algorithmic"], n))
+                if tr else None,
                 "kernel": "gemm_nt_kernel<double,256,128,4,2,TRI=1> (trailing SYRK, rank 256; 128x128 tiles once the "
                           "trailing matrix is below 4096^2)",
                 "avg_launch_us": round(tot_ms / cnt * 1e3, 2), "launches_per_step": cnt // args.steps,

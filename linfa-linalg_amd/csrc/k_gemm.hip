@@ -209,7 +209,8 @@ __device__ inline void tri_tile(int lin, int tiles_m, int tiles_n, int R, int& t
   tm = c / R + rem;
 }
 
-template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool VEC>
+// SQ: the square triangular case (TRI and M == N), i.e. the trailing SYRK.
+template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool SQ, bool VEC>
 __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_kernel(int M, int N, int K, T alpha,
                                                                  const T* __restrict__ A, int64_t lda,
                                                                  const T* __restrict__ B, int64_t ldb, T beta,
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_ker
   const int lin = (id % nx) * per + id / nx;
   if (lin >= total) return;
   int tm, tn;
-  if (TRI && M == N) {
+  if (SQ) {
     tri_tile(lin, tiles_m, tiles_n, BM / BN, tm, tn);
   } else {
     const int per_group = kGroupM * tiles_n;
@@ -247,19 +248,25 @@ template <typename T, int BM, int BN, int WM, int WN, bool TRI>
 hipError_t launch_nt(int M, int N, int K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
                      T* C, int64_t ldc, const int32_t* gate, hipStream_t s) {
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
-  const int total = (TRI && M == N) ? tri_total(tiles_m, tiles_n, BM / BN) : tiles_m * tiles_n;
+  const bool sq = TRI && M == N;
+  const int total = sq ? tri_total(tiles_m, tiles_n, BM / BN) : tiles_m * tiles_n;
   const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
                    ((lda * sizeof(T)) % 16 == 0) && ((ldb * sizeof(T)) % 16 == 0);
   // The kernel deals WG ids to XCDs as lin = (id % 8) * per + id / 8, which is a
   // bijection onto [0, 8 * per) only when the grid has exactly 8 * per WGs.
   const int per = (total + 7) / 8;
   const dim3 grid(8 * per), block(WM * WN * 64);
-  if (vec)
-    hipLaunchKernelGGL((gemm_nt_kernel<T, BM, BN, WM, WN, TRI, true>), grid, block, 0, s, M, N, K, alpha, A, lda,
-                       B, ldb, beta, C, ldc, gate, tiles_m, tiles_n, total);
-  else
-    hipLaunchKernelGGL((gemm_nt_kernel<T, BM, BN, WM, WN, TRI, false>), grid, block, 0, s, M, N, K, alpha, A,
-                       lda, B, ldb, beta, C, ldc, gate, tiles_m, tiles_n, total);
+#define LLH_NT_GO(SQ, VEC)                                                                                  \
+  hipLaunchKernelGGL((gemm_nt_kernel<T, BM, BN, WM, WN, TRI, SQ, VEC>), grid, block, 0, s, M, N, K, alpha, A, lda, \
+                     B, ldb, beta, C, ldc, gate, tiles_m, tiles_n, total)
+  if (sq) {
+    if (vec) LLH_NT_GO(TRI, true);
+    else LLH_NT_GO(TRI, false);
+  } else {
+    if (vec) LLH_NT_GO(false, true);
+    else LLH_NT_GO(false, false);
+  }
+#undef LLH_NT_GO
   return hipGetLastError();
 }
 
