@@ -252,25 +252,9 @@ def rocprof_summary(trace_dir, fetch_dir, write_dir, n, out_md, out_json, comman
         return hits[0]
 
     stats = list(csv.DictReader(open(one(os.path.join(trace_dir, "*kernel_stats.csv")))))
-    # The TRI+SQ instantiation also runs the square inner-panel updates of the
-    # last outer block (below == cols > 64); replay the factorization's launch
-    # order to label each of its launches.
-    pattern = []
-    k0 = 0
-    while k0 < n:
-        kb = min(256, n - k0)
-        for k in range(k0, k0 + kb, 64):
-            b = min(64, k0 + kb - k)
-            below, cols = n - k - b, k0 + kb - (k + b)
-            if below > 0 and cols > 64 and below == cols:
-                pattern.append(False)
-        if n - k0 - kb > 0:
-            pattern.append(True)
-        k0 += kb
-    trace = sorted((r for r in csv.DictReader(open(one(os.path.join(trace_dir, "*kernel_trace.csv"))))
-                    if _is_trailing_syrk(r["Kernel_Name"])), key=lambda r: int(r["Start_Timestamp"]))
-    durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for i, r in enumerate(trace)
-            if pattern[i % len(pattern)]]
+    trace = [r for r in csv.DictReader(open(one(os.path.join(trace_dir, "*kernel_trace.csv"))))
+             if _is_trailing_syrk(r["Kernel_Name"])]
+    durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trace]
     syrk_calls, syrk_ns = len(durs), float(sum(durs))
     counters = {}
     for ctr, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
@@ -279,19 +263,19 @@ def rocprof_summary(trace_dir, fetch_dir, write_dir, n, out_md, out_json, comman
         copy = max(float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"])
         known_kib = n * n * 8 / 1024.0
         factor = known_kib / copy
-        sq = sorted((r for r in rows if _is_trailing_syrk(r["Kernel_Name"])), key=lambda r: int(r["Dispatch_Id"]))
-        vals = [float(r["Counter_Value"]) for i, r in enumerate(sq) if pattern[i % len(pattern)]]
+        vals = [float(r["Counter_Value"]) for r in rows if _is_trailing_syrk(r["Kernel_Name"])]
         counters[ctr] = {"copy_kib": copy, "known_kib": known_kib, "factor": factor, "launches": len(vals),
                          "avg_kib": sum(vals) / len(vals), "bytes_per_launch": sum(vals) / len(vals) * 1024 * factor}
     traffic = counters["FETCH_SIZE"]["bytes_per_launch"] + counters["WRITE_SIZE"]["bytes_per_launch"]
-    # algorithmic bytes of the trailing SYRKs of one factorization (outer block 256)
+    # algorithmic bytes of the trailing SYRKs of one factorization (outer block
+    # 256; with lookahead each SYRK leaves out the next panel's 256 columns)
     alg = []
     k0 = 0
     while k0 < n:
         kb = min(256, n - k0)
-        m2 = n - k0 - kb
-        if m2 > 0:
-            alg.append(8.0 * (m2 * (m2 + 1) + m2 * kb))
+        m3 = n - k0 - kb - min(256, max(n - k0 - kb, 0))
+        if m3 > 0:
+            alg.append(8.0 * (m3 * (m3 + 1) + m3 * kb))
         k0 += kb
     alg_avg = sum(alg) / len(alg)
     res = {"n": n, "kernel_class": "trailing SYRK (gemm_nt_kernel<double,...,TRI,SQ>)",
@@ -312,10 +296,9 @@ def rocprof_summary(trace_dir, fetch_dir, write_dir, n, out_md, out_json, comman
             short = short[:87] + "..."
         lines.append(f"| `{short}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                      f"{float(r['TotalDurationNs']) / 1e6:.1f} | {float(r['Percentage']):.1f} |")
-    lines += ["", f"Trailing SYRK (the TRI+SQ instantiation of `gemm_nt_kernel`, 256x128 tiles while the trailing "
-                  f"matrix is at least 4096^2, 128x128 below; the square inner-panel updates of the last "
-                  f"256-column block, which share the instantiation, left out): {syrk_calls} launches, "
-                  f"average {syrk_ns / syrk_calls / 1e3:.1f} us per launch.", "",
+    lines += ["", f"Trailing SYRK (the TRI+SQ instantiations of `gemm_nt_kernel`, launched only by "
+                  f"`launch_syrk_lower`: 256x128 tiles while the trailing matrix is at least 4096^2, 128x128 "
+                  f"below): {syrk_calls} launches, average {syrk_ns / syrk_calls / 1e3:.1f} us per launch.", "",
               "## HBM traffic per trailing-SYRK launch (`--pmc FETCH_SIZE`, then `--pmc WRITE_SIZE`)", "",
               "| counter | copy of A reads (KiB) | known (KiB) | factor | avg per launch (KiB) | bytes per launch |",
               "|---|---:|---:|---:|---:|---:|"]

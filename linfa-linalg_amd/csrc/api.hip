@@ -99,6 +99,21 @@ hipError_t ctx_dinv(llh_context* ctx, int n, size_t es, void** out) {
   return e;
 }
 
+const Lookahead* ctx_lookahead(llh_context* ctx) {
+  if (ctx->la.side == nullptr) {
+    Lookahead la;
+    if (hipStreamCreateWithFlags(&la.side, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&la.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&la.join, hipEventDisableTiming) != hipSuccess) {
+      if (la.fork) (void)hipEventDestroy(la.fork);
+      (void)hipStreamDestroy(la.side);
+      return nullptr;
+    }
+    ctx->la = la;
+  }
+  return &ctx->la;
+}
+
 hipError_t ctx_sync(llh_context* ctx, int n, int m, SolveSync* sync) {
   const int64_t nb = (n + kNB - 1) / kNB, nq = (m + kSolveChunkHost - 1) / kSolveChunkHost;
   const int64_t nflags = nb * std::max<int64_t>(nq, 1);
@@ -186,7 +201,8 @@ llh_status factor_plain(llh_context* ctx, size_t n, T* a, ptrdiff_t rs, ptrdiff_
   void* dinv = nullptr;
   CHECK_HIP(ctx_dinv(ctx, (int)n, sizeof(T), &dinv), "inverse buffer");
   CHECK_HIP(hipMemcpyAsync(d, h.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
-  CHECK_HIP(potrf_enqueue<T>(d, (int64_t)n, (int)n, ctx->d_info, clean, static_cast<T*>(dinv), ctx->stream),
+  CHECK_HIP(potrf_enqueue<T>(d, (int64_t)n, (int)n, ctx->d_info, clean, static_cast<T*>(dinv), ctx->stream, nullptr,
+                            false, ctx_lookahead(ctx)),
             "potrf");
   CHECK_HIP(hipMemcpyAsync(h.data(), d, n * n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
@@ -337,7 +353,8 @@ llh_status solve_on_device(llh_context* ctx, size_t n, T* da, size_t m, T* db, l
     T* dinv = static_cast<T*>(dv);
     SolveSync sync;
     CHECK_HIP(ctx_sync(ctx, (int)n, (int)m, &sync), "solve sync");
-    CHECK_HIP(potrf_enqueue<T>(da, (int64_t)n, (int)n, ctx->d_info, false, dinv, ctx->stream, ctx->profiler()),
+    CHECK_HIP(potrf_enqueue<T>(da, (int64_t)n, (int)n, ctx->d_info, false, dinv, ctx->stream, ctx->profiler(),
+                                               false, ctx_lookahead(ctx)),
               "potrf");
     CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, dinv, false, nullptr,
                               &sync, ctx->stream, ctx->profiler()),
@@ -425,7 +442,7 @@ llh_status dev_potrf(llh_context* ctx, size_t n, T* a, size_t lda, int zero_uppe
   void* dinv = nullptr;
   CHECK_HIP(ctx_dinv(ctx, (int)n, sizeof(T), &dinv), "inverse buffer");
   CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, zero_upper != 0, static_cast<T*>(dinv),
-                            ctx->stream, ctx->profiler()),
+                            ctx->stream, ctx->profiler(), false, ctx_lookahead(ctx)),
             "potrf");
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
             "status");
@@ -465,7 +482,8 @@ llh_status dev_solvec(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T*
   T* dinv = static_cast<T*>(dv);
   SolveSync sync;
   CHECK_HIP(ctx_sync(ctx, (int)n, (int)std::max<size_t>(m, 1), &sync), "solve sync");
-  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, false, dinv, ctx->stream, ctx->profiler()),
+  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, false, dinv, ctx->stream, ctx->profiler(), false,
+                            ctx_lookahead(ctx)),
             "potrf");
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
             "status");
@@ -537,6 +555,12 @@ void llh_context_destroy(llh_context* ctx) {
   if (ctx->sync.ptr) (void)hipFree(ctx->sync.ptr);
   if (ctx->d_info) (void)hipFree(ctx->d_info);
   if (ctx->h_info) (void)hipHostFree(ctx->h_info);
+  if (ctx->la.side) {
+    (void)hipStreamSynchronize(ctx->la.side);
+    (void)hipEventDestroy(ctx->la.fork);
+    (void)hipEventDestroy(ctx->la.join);
+    (void)hipStreamDestroy(ctx->la.side);
+  }
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->magic = 0;
   delete ctx;

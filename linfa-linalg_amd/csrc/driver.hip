@@ -13,23 +13,71 @@ namespace llh {
     if (e_ != hipSuccess) return e_;   \
   } while (0)
 
-// Brackets one launch with the profiler's events (no-op when profiling is off).
-#define LLH_PROF(cls, flops, bytes, expr)                                       \
+// Brackets one launch on stream st with the profiler's events (no-op when
+// profiling is off).
+#define LLH_PROF_ON(st, cls, flops, bytes, expr)                                \
   do {                                                                          \
-    const int pi_ = prof ? prof->begin((cls), (flops), (bytes), s) : -1;        \
+    const int pi_ = prof ? prof->begin((cls), (flops), (bytes), (st)) : -1;     \
     LLH_TRY(expr);                                                              \
-    if (prof) prof->end(pi_, s);                                                \
+    if (prof) prof->end(pi_, (st));                                             \
   } while (0)
+#define LLH_PROF(cls, flops, bytes, expr) LLH_PROF_ON(s, cls, flops, bytes, expr)
+
+namespace {
+
+// Factors the outer panel of columns [k0, k0 + kb) on stream st, kNB columns at
+// a time: the diagonal block in LDS (with its inverse), the rows below it as one
+// MFMA product with that inverse, then the rest of the outer panel.  Everything
+// left of k0 has already been applied to these columns.
+template <typename T>
+hipError_t factor_panel(T* a, int64_t lda, int n, int k0, int kb, int32_t* info, T* dinv, hipStream_t st,
+                        Profiler* prof, bool diag_f64) {
+  const double es = (double)sizeof(T);
+  for (int k = k0; k < k0 + kb; k += kNB) {
+    const int b = std::min(kNB, k0 + kb - k);
+    T* akk = a + (int64_t)k * lda + k;
+    const double db = b;
+    T* dk = dinv + (int64_t)(k / kNB) * kNB * kNB;
+    if (diag_f64 && sizeof(T) == 4) {
+      LLH_PROF_ON(st, kDiagFactor, 2.0 * db * db * db / 3.0, es * db * db * 2.0,
+                  (launch_diag_fast<T, double>(akk, lda, b, dk, info, k, info, st)));
+    } else {
+      LLH_PROF_ON(st, kDiagFactor, 2.0 * db * db * db / 3.0, es * db * db * 2.0,
+                  (launch_diag_fast<T>(akk, lda, b, dk, info, k, info, st)));
+    }
+    const int below = n - k - b;
+    if (below <= 0) continue;
+    T* panel = akk + (int64_t)b * lda;  // rows k+b.., columns k..k+b
+    // L21 = A21 * L11^-T as one MFMA product with the inverse, in place (each
+    // workgroup reads all of its rows' k before writing them).
+    LLH_PROF_ON(st, kPanelTrsm, (double)below * db * db * 2.0, es * ((double)below * db * 2.0 + db * db),
+                (launch_gemm<T>(false, true, false, below, b, b, T(1), panel, lda, dk, kNB, T(0), panel, lda, info,
+                                st)));
+    const int cols = k0 + kb - (k + b);
+    if (cols > 0) {
+      // rows below x cols, lower part of the top cols x cols square only
+      const double outs = (double)below * cols - (double)cols * (cols - 1) / 2.0;
+      LLH_PROF_ON(st, kPanelUpdate, 2.0 * outs * db, es * (2.0 * outs + ((double)below + cols) * db),
+                  (launch_gemm<T>(false, true, true, below, cols, b, T(-1), panel, lda, panel, lda, T(1), panel + b,
+                                  lda, info, st)));
+    }
+  }
+  return hipSuccess;
+}
+
+}  // namespace
 
 // Right-looking blocked Cholesky of the lower triangle (the blocked equivalent
-// of cholesky.rs:54-74).  For each kNBO-wide outer panel, kNB-wide inner steps
-// factor the diagonal block in LDS, solve the rows below it, and update the rest
-// of the outer panel; then one rank-kNBO SYRK on MFMA updates the trailing
-// matrix.  Matrices no larger than the LDS kernel's limit are factored by that
-// kernel alone, in the reference's exact operation order.
+// of cholesky.rs:54-74).  Each kNBO-wide outer panel is factored by
+// factor_panel; then one rank-kNBO SYRK on MFMA updates the trailing matrix.
+// With lookahead, the trailing update of panel k is split: the next panel's
+// columns are updated and that panel factored on the side stream while the SYRK
+// of the remaining trailing matrix runs on s.  Matrices no larger than the LDS
+// kernel's limit are factored by that kernel alone, in the reference's exact
+// operation order.
 template <typename T>
 hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, T* dinv, hipStream_t s,
-                         Profiler* prof, bool diag_f64) {
+                         Profiler* prof, bool diag_f64, const Lookahead* la) {
   const double es = (double)sizeof(T);
   LLH_TRY(launch_fill_i32(info, kNoFail, 1, s));
   if (n <= 0) return hipSuccess;
@@ -38,50 +86,44 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
     LLH_TRY(launch_potrf_exact_single<T>(a, lda, n, info, 0, nullptr, s));
     if (dinv != nullptr) LLH_TRY(launch_trtri_blocks<T>(n, a, lda, false, false, dinv, info, s));
   } else {
+    LLH_TRY(factor_panel<T>(a, lda, n, 0, std::min(kNBO, n), info, dinv, s, prof, diag_f64));
     for (int k0 = 0; k0 < n; k0 += kNBO) {
       const int kb = std::min(kNBO, n - k0);
-      for (int k = k0; k < k0 + kb; k += kNB) {
-        const int b = std::min(kNB, k0 + kb - k);
-        T* akk = a + (int64_t)k * lda + k;
-        const double db = b;
-        // factor + inverse of the diagonal block (the inverse serves this
-        // panel's solve and, later, the triangular solves)
-        T* dk = dinv + (int64_t)(k / kNB) * kNB * kNB;
-        if (diag_f64 && sizeof(T) == 4) {
-          LLH_PROF(kDiagFactor, 2.0 * db * db * db / 3.0, es * db * db * 2.0,
-                   (launch_diag_fast<T, double>(akk, lda, b, dk, info, k, info, s)));
-        } else {
-          LLH_PROF(kDiagFactor, 2.0 * db * db * db / 3.0, es * db * db * 2.0,
-                   (launch_diag_fast<T>(akk, lda, b, dk, info, k, info, s)));
-        }
-        const int below = n - k - b;
-        if (below <= 0) continue;
-        T* panel = akk + (int64_t)b * lda;  // rows k+b.., columns k..k+b
-        // L21 = A21 * L11^-T as one MFMA product with the inverse, in place (each
-        // workgroup reads all of its rows' k before writing them).
-        LLH_PROF(kPanelTrsm, (double)below * db * db * 2.0, es * ((double)below * db * 2.0 + db * db),
-                 (launch_gemm<T>(false, true, false, below, b, b, T(1), panel, lda, dk, kNB, T(0), panel, lda,
-                                 info, s)));
-        const int cols = k0 + kb - (k + b);
-        if (cols > 0) {
-          // rows below x cols, lower part of the top cols x cols square only
-          const double outs = (double)below * cols - (double)cols * (cols - 1) / 2.0;
-          LLH_PROF(kPanelUpdate, 2.0 * outs * db, es * (2.0 * outs + ((double)below + cols) * db),
-                   (launch_gemm<T>(false, true, true, below, cols, b, T(-1), panel, lda, panel, lda, T(1),
-                                   panel + b, lda, info, s)));
-        }
-      }
       const int m2 = n - k0 - kb;
-      if (m2 > 0) {
-        T* p = a + (int64_t)(k0 + kb) * lda + k0;
-        // Algorithmic work of the lower-triangle SYRK: m2(m2+1)/2 outputs, each a
-        // kb-term dot product (2 flops per term); bytes: read and write the lower
-        // triangle of C once, read the m2 x kb panel once.
+      if (m2 <= 0) break;
+      const int kb2 = std::min(kNBO, m2);  // width of the next panel
+      T* p = a + (int64_t)(k0 + kb) * lda + k0;  // L21: rows k0+kb.., columns k0..k0+kb
+      T* c = p + kb;                              // trailing matrix at (k0+kb, k0+kb)
+      // The trailing SYRK's algorithmic work: outs outputs of the lower
+      // triangle, each a kb-term dot product; bytes: read and write the lower
+      // triangle of C once, read the panel once.
+      if (la == nullptr) {
         const double outs = (double)m2 * (m2 + 1) / 2.0;
         LLH_PROF(kTrailingSyrk, 2.0 * outs * kb, es * (2.0 * outs + (double)m2 * kb),
-                 (launch_gemm<T>(false, true, true, m2, m2, kb, T(-1), p, lda, p, lda, T(1), p + kb, lda,
-                                 info, s)));
+                 (launch_syrk_lower<T>(m2, kb, T(-1), p, lda, T(1), c, lda, info, s)));
+        LLH_TRY(factor_panel<T>(a, lda, n, k0 + kb, kb2, info, dinv, s, prof, diag_f64));
+        continue;
       }
+      LLH_TRY(hipEventRecord(la->fork, s));
+      LLH_TRY(hipStreamWaitEvent(la->side, la->fork, 0));
+      {
+        // next panel's columns (m2 rows x kb2 columns, lower part of the top
+        // square), then its factor, on the side stream
+        const double outs = (double)m2 * kb2 - (double)kb2 * (kb2 - 1) / 2.0;
+        LLH_PROF_ON(la->side, kPanelUpdate, 2.0 * outs * kb, es * (2.0 * outs + ((double)m2 + kb2) * kb),
+                    (launch_gemm<T>(false, true, true, m2, kb2, kb, T(-1), p, lda, p, lda, T(1), c, lda, info,
+                                    la->side)));
+        LLH_TRY(factor_panel<T>(a, lda, n, k0 + kb, kb2, info, dinv, la->side, prof, diag_f64));
+      }
+      LLH_TRY(hipEventRecord(la->join, la->side));
+      const int m3 = m2 - kb2;  // the rest of the trailing matrix
+      if (m3 > 0) {
+        const double outs = (double)m3 * (m3 + 1) / 2.0;
+        T* p3 = p + (int64_t)kb2 * lda;
+        LLH_PROF(kTrailingSyrk, 2.0 * outs * kb, es * (2.0 * outs + (double)m3 * kb),
+                 (launch_syrk_lower<T>(m3, kb, T(-1), p3, lda, T(1), c + (int64_t)kb2 * lda + kb2, lda, info, s)));
+      }
+      LLH_TRY(hipStreamWaitEvent(s, la->join, 0));
     }
   }
   if (zero_upper) LLH_TRY(launch_zero_triangle<T>(a, lda, n, true, info, s));
@@ -148,9 +190,9 @@ hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t ld
 }
 
 template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool, double*, hipStream_t,
-                                          Profiler*, bool);
+                                          Profiler*, bool, const Lookahead*);
 template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, float*, hipStream_t, Profiler*,
-                                         bool);
+                                         bool, const Lookahead*);
 template hipError_t trsm_enqueue<double>(bool, bool, int, const double*, int64_t, int, double*, int64_t,
                                          const double*, bool, double*, SolveSync*, hipStream_t, Profiler*);
 template hipError_t trsm_enqueue<float>(bool, bool, int, const float*, int64_t, int, float*, int64_t,

@@ -241,15 +241,21 @@ __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_ker
   const int m0 = tm * BM, n0 = tn * BN;
   if (TRI && m0 + BM - 1 < n0) return;
 
-  nt_tile<T, BM, BN, WM, WN, TRI, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
+  // C prefetch for the f64 trailing SYRK on 256 x 128 tiles (measured: it
+  // slows the f32 tiles, and the 128 x 128 f64 tile has no registers left)
+  constexpr bool PRE = SQ && sizeof(T) == 8 && BM == 256;
+  nt_tile<T, BM, BN, WM, WN, TRI, VEC, PRE>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
 }
 
-template <typename T, int BM, int BN, int WM, int WN, bool TRI>
+// SQ launches the square lower-triangle form (M == N, TRI), which only the
+// trailing SYRK uses (launch_syrk_lower), so that it is its own instantiation.
+template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool SQ = false>
 hipError_t launch_nt(int M, int N, int K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
                      T* C, int64_t ldc, const int32_t* gate, hipStream_t s) {
+  static_assert(!SQ || TRI, "the square form is triangular");
+  if (SQ && M != N) return hipErrorInvalidValue;
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
-  const bool sq = TRI && M == N;
-  const int total = sq ? tri_total(tiles_m, tiles_n, BM / BN) : tiles_m * tiles_n;
+  const int total = SQ ? tri_total(tiles_m, tiles_n, BM / BN) : tiles_m * tiles_n;
   const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
                    ((lda * sizeof(T)) % 16 == 0) && ((ldb * sizeof(T)) % 16 == 0);
   // The kernel deals WG ids to XCDs as lin = (id % 8) * per + id / 8, which is a
@@ -259,13 +265,8 @@ hipError_t launch_nt(int M, int N, int K, T alpha, const T* A, int64_t lda, cons
 #define LLH_NT_GO(SQ, VEC)                                                                                  \
   hipLaunchKernelGGL((gemm_nt_kernel<T, BM, BN, WM, WN, TRI, SQ, VEC>), grid, block, 0, s, M, N, K, alpha, A, lda, \
                      B, ldb, beta, C, ldc, gate, tiles_m, tiles_n, total)
-  if (sq) {
-    if (vec) LLH_NT_GO(TRI, true);
-    else LLH_NT_GO(TRI, false);
-  } else {
-    if (vec) LLH_NT_GO(false, true);
-    else LLH_NT_GO(false, false);
-  }
+  if (vec) LLH_NT_GO(SQ, true);
+  else LLH_NT_GO(SQ, false);
 #undef LLH_NT_GO
   return hipGetLastError();
 }
@@ -300,7 +301,8 @@ hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha,
   if (!ta && tb && N > 64) {
     // Large updates: 256 x 128 tiles on 8 waves (2 per SIMD) cut the operand
     // traffic per flop by a quarter; smaller ones keep 128 x 128 tiles so the
-    // grid still fills the 256 CUs.
+    // grid still fills the 256 CUs.  Triangular updates enumerate the full
+    // rectangle; tiles wholly above the diagonal exit at once.
     const bool big = (int64_t)M * N >= (int64_t)4096 * 4096;
     if (tri) {
       if (big) return launch_nt<T, 256, 128, 4, 2, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
@@ -319,6 +321,23 @@ hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha,
     return launch_shape<T, true, false, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
   return hipErrorInvalidValue;
 }
+
+template <typename T>
+hipError_t launch_syrk_lower(int n, int k, T alpha, const T* A, int64_t lda, T beta, T* C, int64_t ldc,
+                             const int32_t* gate, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (k < 0) k = 0;
+  // 256 x 128 tiles on 8 waves (2 per SIMD) while the grid still fills the chip
+  // several times over; 128 x 128 tiles for the last few small updates.
+  if ((int64_t)n * n >= (int64_t)4096 * 4096)
+    return launch_nt<T, 256, 128, 4, 2, true, true>(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, gate, s);
+  return launch_nt<T, 128, 128, 2, 2, true, true>(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, gate, s);
+}
+
+template hipError_t launch_syrk_lower<double>(int, int, double, const double*, int64_t, double, double*, int64_t,
+                                              const int32_t*, hipStream_t);
+template hipError_t launch_syrk_lower<float>(int, int, float, const float*, int64_t, float, float*, int64_t,
+                                             const int32_t*, hipStream_t);
 
 template hipError_t launch_gemm<double>(bool, bool, bool, int, int, int, double, const double*, int64_t,
                                         const double*, int64_t, double, double*, int64_t, const int32_t*,

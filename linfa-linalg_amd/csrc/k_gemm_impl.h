@@ -25,6 +25,7 @@ struct Mfma<double> {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int lane, int i) { return 4 * i + (lane >> 4); }
+  static constexpr int kRowStep = 4;  // row(lane, i) = row(lane, 0) + kRowStep * i
 };
 template <>
 struct Mfma<float> {
@@ -33,6 +34,7 @@ struct Mfma<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
+  static constexpr int kRowStep = 1;
 };
 
 // One BM x BN tile of C = beta*C + alpha*A*B^T on WM x WN waves.
@@ -50,7 +52,14 @@ struct Mfma<float> {
 // carry no bounds checks; their products land in rows the epilogue discards.
 // Only a final partial slab of k is loaded with masking.  A may alias C only when
 // each tile's rows of A are its own rows of C (every read precedes the writes).
-template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool VEC>
+//
+// PRE (beta != 0, alpha != 0, at least FM*FN/CPS slabs): C is read during the
+// main loop instead of after it.  The last FM*FN/CPS slabs each issue the loads
+// of CPS accumulator fragments of C, and the next slab folds them in as
+// acc += (beta/alpha) C, so the epilogue only stores alpha * acc.  Without it
+// every workgroup of a rank-k update with small k reads its C tile at the same
+// moment, after its MFMAs, and the launch waits on one HBM burst per tile.
+template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool VEC, bool PRE = false>
 __device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const T* A, int64_t lda,
                                         const T* __restrict__ B, int64_t ldb, T beta, T* C, int64_t ldc,
                                         int m0, int n0) {
@@ -117,14 +126,46 @@ __device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const T* A
   };
 
   const int nslab = (K + SK - 1) / SK;
+  const int fr = lane & 15, fq = (lane >> 4) * KPL;
+
+  // C prefetch: fragment q = (i, j) -> group q / CPS, loaded in slab pre0 + group
+  constexpr int NF = FM * FN;
+  constexpr int CPS = sizeof(T) == 8 ? 1 : 2;  // fragments per slab (8 VGPRs each)
+  constexpr int NG = NF / CPS;
+  static_assert(NF % CPS == 0, "fragments split evenly over slabs");
+  const bool pre = PRE && alpha != T(0) && beta != T(0) && nslab >= NG;
+  const int pre0 = nslab - NG;
+  const T cscale = pre ? beta / alpha : T(0);
+  acc_t cbuf[CPS];
+  const int ci0 = m0 + wm0 + MF::row(lane, 0), cj0 = n0 + wn0 + fr;
+  const T* cbase = C + (int64_t)ci0 * ldc + cj0;
+  auto cload = [&](int g) {
+#pragma unroll
+    for (int c = 0; c < CPS; ++c) {
+      const int q = g * CPS + c, i = q / FN, j = q % FN;  // uniform
+      const int gj = cj0 + j * 16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int di = i * 16 + MF::kRowStep * r, gi = ci0 + di;
+        cbuf[c][r] = (gi < M && gj < N && (!TRI || gi >= gj)) ? cbase[(int64_t)di * ldc + j * 16] : T(0);
+      }
+    }
+  };
+  auto cadd = [&](int g) {
+#pragma unroll
+    for (int q = 0; q < NF; ++q)
+      if (q / CPS == g) acc[q / FN][q % FN] += cscale * cbuf[q % CPS];
+  };
+
   load(0);
   store(0);
   __syncthreads();
-  const int fr = lane & 15, fq = (lane >> 4) * KPL;
   for (int sl = 0; sl < nslab; ++sl) {
     const int cur = sl & 1;
     const bool more = sl + 1 < nslab;
+    if (PRE && pre && sl > pre0) cadd(sl - 1 - pre0);
     if (more) load((sl + 1) * SK);
+    if (PRE && pre && sl >= pre0) cload(sl - pre0);
     const T* as = As[cur];
     const T* bs = Bs[cur];
 #pragma unroll
@@ -144,6 +185,7 @@ __device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const T* A
     if (more) store(cur ^ 1);
     __syncthreads();
   }
+  if (PRE && pre) cadd(NG - 1);
 
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -156,7 +198,7 @@ __device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const T* A
         if (gi < M && gj < N && (!TRI || gi >= gj)) {
           T* c = C + (int64_t)gi * ldc + gj;
           const T v = alpha * acc[i][j][r];
-          *c = (beta == T(0)) ? v : beta * *c + v;
+          *c = (beta == T(0) || (PRE && pre)) ? v : beta * *c + v;
         }
       }
 }

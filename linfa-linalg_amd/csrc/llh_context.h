@@ -27,6 +27,7 @@ struct llh_context {
   Buffer mixed_work;  // its residual / correction / norm buffers
   Buffer sync;        // ticket / error / flags of the one-launch solve
   int32_t epoch = 1;
+  llh::Lookahead la;  // side stream and events of the factor's lookahead (created on first use)
   llh::Profiler prof;  // live kernel timing (off unless enabled)
   llh::Profiler* profiler() { return prof.on ? &prof : nullptr; }
 };
@@ -45,6 +46,9 @@ hipError_t ctx_buffer(llh_context* ctx, int slot, size_t bytes, void** out);
 hipError_t ctx_dinv(llh_context* ctx, int n, size_t es, void** out);
 // Solve-sync words sized for order n and m right-hand sides; fills *sync.
 hipError_t ctx_sync(llh_context* ctx, int n, int m, SolveSync* sync);
+// The lookahead streams of the blocked factor, created on first use (nullptr if
+// they cannot be created: the factor then runs on the context stream alone).
+const Lookahead* ctx_lookahead(llh_context* ctx);
 // Enqueues the read-back of the solve error word into h_info[2].
 hipError_t ctx_read_solve_error(llh_context* ctx);
 }  // namespace llh

@@ -18,6 +18,14 @@ struct SolveSync {
 // Number of elements of the diagonal-block inverse buffer for order n.
 inline int64_t dinv_elems(int n) { return (int64_t)((n + kNB - 1) / kNB) * kNB * kNB; }
 
+// Second stream and a fork/join event pair for the factor's lookahead: while
+// the trailing SYRK of outer panel k runs on the caller's stream, the next
+// panel's columns are updated and factored on `side`.  All default priority.
+struct Lookahead {
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
 // Enqueues the in-place lower Cholesky of a (row-major, lda) on stream s.
 // *info receives kNoFail or the first row whose pivot fails `d <= 0`.  When
 // dinv is non-null it receives inv(L_ii) of every 64 x 64 diagonal block (the
@@ -25,9 +33,11 @@ inline int64_t dinv_elems(int n) { return (int64_t)((n + kNB - 1) / kNB) * kNB *
 // later triangular solves).
 // diag_f64 (f32 storage only): the diagonal blocks are factored and inverted in
 // f64 arithmetic, the mixed-precision form.
+// la (optional): lookahead streams; the whole factor is still complete in
+// stream order on s.
 template <typename T>
 hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, T* dinv, hipStream_t s,
-                         Profiler* prof = nullptr, bool diag_f64 = false);
+                         Profiler* prof = nullptr, bool diag_f64 = false, const Lookahead* la = nullptr);
 
 // Enqueues op(A) X = B in place (B row-major n x m); upper_op names the triangle
 // of op(A) and trans != 0 means op(A) = A^T.  dinv, when non-null, holds the

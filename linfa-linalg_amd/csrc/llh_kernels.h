@@ -114,6 +114,12 @@ hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha,
                        const T* B, int64_t ldb, T beta, T* C, int64_t ldc, const int32_t* gate,
                        hipStream_t s);
 
+// k_gemm.hip: C = beta*C + alpha*A*A^T on the lower triangle of the n x n C
+// (A is n x k, k-contiguous rows): the trailing update of the blocked POTRF.
+template <typename T>
+hipError_t launch_syrk_lower(int n, int k, T alpha, const T* A, int64_t lda, T beta, T* C, int64_t ldc,
+                             const int32_t* gate, hipStream_t s);
+
 // k_trsm.hip: panel solve X := X * L^-T (X is m x b, L is b x b lower, b <= kNB).
 template <typename T>
 hipError_t launch_trsm_rows(int m, int b, T* X, int64_t ldx, const T* L, int64_t ldl, const int32_t* gate,

@@ -88,7 +88,7 @@ llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error
       float* l = static_cast<float*>(f->l);
       CHECK_HIP(launch_convert_d2s(n, n, f->a, f->lda, l, n, true, s), "convert");
       CHECK_HIP(potrf_enqueue<float>(l, n, n, ctx->d_info, false, static_cast<float*>(f->dinv), s,
-                                     ctx->profiler(), true),
+                                     ctx->profiler(), true, ctx_lookahead(ctx)),
                 "potrf f32");
     } else {
       double* l = static_cast<double*>(f->l);
@@ -96,7 +96,7 @@ llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error
                                  hipMemcpyDeviceToDevice, s),
                 "copy");
       CHECK_HIP(potrf_enqueue<double>(l, n, n, ctx->d_info, false, static_cast<double*>(f->dinv), s,
-                                      ctx->profiler()),
+                                      ctx->profiler(), false, ctx_lookahead(ctx)),
                 "potrf f64");
     }
     CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, s), "status");
