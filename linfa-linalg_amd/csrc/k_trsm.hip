@@ -127,27 +127,38 @@ __global__ __launch_bounds__(256) void trtri64_kernel(int n, const TS* __restric
 //
 // Workgroups take tickets in launch order; ticket t is (position p, column chunk
 // q).  Position p is block row w = p (forward, op(A) lower) or w = nb-1-p
-// (backward).  The workgroup accumulates acc = B_w - sum_j op(A)_wj X_j over the
-// blocks j that precede w in solve order, waiting for each X_j on its flag, then
-// forms X_w = inv(op(A)_ww) acc with the precomputed inverse, stores it and
-// raises flag (w, q).  A workgroup only ever waits for positions taken before
-// its own, so the chain always advances whatever the residency.  Flags hold the
-// epoch of the call that raised them, so they never need clearing.  The wait is
-// bounded: past spin_limit polls (or once another workgroup has given up) the
-// workgroup records an error and returns without raising its flag.
+// (backward).  The workgroup accumulates sum_j op(A)_wj X_j over the blocks j
+// that precede w in solve order, then forms X_w = inv(op(A)_ww) (B_w - sum) with
+// the precomputed inverse, stores it and raises flag (w, q).  A workgroup only
+// ever waits for positions taken before its own, so the chain always advances
+// whatever the residency.  Flags hold the epoch of the call that raised them, so
+// they never need clearing.
+//
+// The walk is throughput-bound for every workgroup behind the frontier, so it
+// keeps synchronisation off the per-block path: wave v owns columns
+// [16v, 16v + 16) of each 64 x 64 tile, and each lane (row r) holds its 16 tile
+// values in registers, loaded from global memory one block ahead; X_j goes
+// through a double-buffered LDS slab with one barrier per block; and flags are
+// checked in windows: one poll and one acquire fence cover every block whose
+// flag is already raised.  The wave partial sums are reduced once, at the end.
+// The wait is bounded: past spin_limit polls (or once another workgroup has
+// given up) the workgroup records an error and returns without raising its flag.
 constexpr int kSolveChunk = kSolveChunkHost;
+constexpr int kFlagWindow = 64;  // flags checked ahead per wait
 
-template <typename T>
-__global__ __launch_bounds__(256) void trsm_persistent_kernel(int n, int m, const T* __restrict__ A,
-                                                              int64_t lda, bool trans, bool lower_op,
-                                                              const T* __restrict__ Dinv, bool dinv_t,
-                                                              T* B, int64_t ldb, int32_t* ticket,
-                                                              int32_t* flags, int32_t epoch, int32_t* err,
-                                                              int64_t spin_limit) {
-  constexpr int S = kNB + 1, MC = kSolveChunk, SC = MC + 1;
-  __shared__ T As[kNB * S];
-  __shared__ T Ys[kNB * SC];
-  __shared__ int s_ticket, s_abort;
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256, 2) void trsm_persistent_kernel(int n, int m, const T* __restrict__ A,
+                                                                 int64_t lda, bool trans, bool lower_op,
+                                                                 const T* __restrict__ Dinv, bool dinv_t,
+                                                                 T* B, int64_t ldb, int32_t* ticket,
+                                                                 int32_t* flags, int32_t epoch, int32_t* err,
+                                                                 int64_t spin_limit) {
+  constexpr int MC = kSolveChunk, KW = kNB / 4, S = kNB + 1;
+  static_assert(MC == 16 && KW == 16, "one 16-column chunk, 16 tile columns per wave");
+  __shared__ __attribute__((aligned(16))) T Ys[2][kNB * MC];  // X_j slabs (row k, column c)
+  __shared__ __attribute__((aligned(16))) T Part[3][kNB * MC];  // partial sums of waves 1..3
+  __shared__ T Dl[kNB * S];                                     // inv(op(A)_ww)
+  __shared__ int s_ticket, s_ready, s_abort;
   const int tid = threadIdx.x;
   const int nb = (n + kNB - 1) / kNB, nq = (m + MC - 1) / MC;
   if (tid == 0) {
@@ -161,89 +172,138 @@ __global__ __launch_bounds__(256) void trsm_persistent_kernel(int n, int m, cons
   const int w = lower_op ? p : nb - 1 - p;
   const int w0 = w * kNB, bw = min(kNB, n - w0);
   const int c0 = q * MC, mc = min(MC, m - c0);
-  // This thread's outputs: row r, columns cc .. cc+3 of the chunk.
-  const int r = tid >> 2, cc = (tid & 3) * 4;
-  T acc[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e)
-    acc[e] = (r < bw && cc + e < mc) ? B[(int64_t)(w0 + r) * ldb + c0 + cc + e] : T(0);
+  const int r = tid & 63, wv = tid >> 6;
+  auto jof = [&](int st) { return lower_op ? st : nb - 1 - st; };
 
-  // op(A)[w0+r, j0+k] is A[w0+r][j0+k], or A[j0+k][w0+r] when trans.  Block s
-  // of the walk is prefetched into registers one step ahead (it does not depend
-  // on any flag), element e = tid + 256 g being (i, jj) = (e / 64, e % 64) of the
-  // stored tile, read coalesced over jj.
-  T pre[16];
-  auto fetch = [&](int s) {
-    const int j = lower_op ? s : nb - 1 - s;
-    const int j0 = j * kNB, bj = min(kNB, n - j0);
+  // the inverse of the diagonal block does not depend on any flag: stage it first
+  {
+    const T* D = Dinv + (int64_t)w * kNB * kNB;
 #pragma unroll
     for (int g = 0; g < 16; ++g) {
       const int e = tid + 256 * g, i = e / kNB, jj = e % kNB;
-      const int rr = trans ? jj : i, kk = trans ? i : jj;
-      pre[g] = (rr < bw && kk < bj)
-                   ? (trans ? A[(int64_t)(j0 + i) * lda + w0 + jj] : A[(int64_t)(w0 + i) * lda + j0 + jj])
-                   : T(0);
+      Dl[(dinv_t ? jj : i) * S + (dinv_t ? i : jj)] = D[e];
+    }
+  }
+
+  // this lane's 16 values of op(A)[w0 + r, j0 + 16 wv + kk] for block j
+  typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+  constexpr int VL = 16 / sizeof(T);
+  auto fetch = [&](int st, T* out) {
+    const int j = jof(st), j0 = j * kNB, bj = min(kNB, n - j0), k0 = KW * wv;
+    if (!trans) {
+      const T* row = A + (int64_t)(w0 + min(r, bw - 1)) * lda + j0 + k0;
+      if (VEC && k0 + KW <= bj) {
+#pragma unroll
+        for (int v = 0; v < KW / VL; ++v) {
+          const vec_t x = *reinterpret_cast<const vec_t*>(row + v * VL);
+#pragma unroll
+          for (int e = 0; e < VL; ++e) out[v * VL + e] = x[e];
+        }
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < KW; ++kk) out[kk] = (k0 + kk < bj) ? row[kk] : T(0);
+      }
+    } else {
+      const T* col = A + (int64_t)j0 * lda + w0 + min(r, bw - 1);
+#pragma unroll
+      for (int kk = 0; kk < KW; ++kk) out[kk] = (k0 + kk < bj) ? col[(int64_t)(k0 + kk) * lda] : T(0);
     }
   };
-  if (p > 0) fetch(0);
-  for (int s = 0; s < p; ++s) {
-    const int j = lower_op ? s : nb - 1 - s;
-    const int j0 = j * kNB, bj = min(kNB, n - j0);
+  // X_j rows [0, 64) x chunk columns: 4 values per thread (row e / 16, column e % 16)
+  auto loady = [&](int st, T* out) {
+    const int j0 = jof(st) * kNB, bj = min(kNB, n - j0);
 #pragma unroll
-    for (int g = 0; g < 16; ++g) {
-      const int e = tid + 256 * g, i = e / kNB, jj = e % kNB;
-      As[(trans ? jj : i) * S + (trans ? i : jj)] = pre[g];
+    for (int g = 0; g < 4; ++g) {
+      const int e = tid + 256 * g, k = e / MC, c = e % MC;
+      out[g] = (k < bj && c < mc) ? B[(int64_t)(j0 + k) * ldb + c0 + c] : T(0);
     }
-    if (s + 1 < p) fetch(s + 1);
-    if (tid == 0) {
-      // Poll with relaxed loads (no cache maintenance per poll); one acquire
-      // fence once the flag is seen.
-      int64_t it = 0;
-      while (__hip_atomic_load(&flags[j * nq + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (++it > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-          atomicExch(err, 1);
-          s_abort = 1;
-          break;
+  };
+
+  T acc[MC];
+#pragma unroll
+  for (int c = 0; c < MC; ++c) acc[c] = T(0);
+  T a_cur[KW], a_nxt[KW], y_cur[4], y_nxt[4];
+  int ready = 0;
+  bool y_pre = false;
+  if (p > 0) fetch(0, a_cur);
+  for (int st = 0; st < p; ++st) {
+    if (st >= ready) {
+      // wait for block st's flag, then take every later flag already raised
+      if (tid == 0) {
+        int64_t it = 0;
+        const int32_t* f = &flags[jof(st) * nq + q];
+        while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+          if (++it > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            atomicExch(err, 1);
+            s_abort = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_s_sleep(1);
+        int t2 = st + 1;
+        while (t2 < p && t2 < st + kFlagWindow &&
+               __hip_atomic_load(&flags[jof(t2) * nq + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+          ++t2;
+        s_ready = t2;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __syncthreads();
+      if (s_abort) return;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      ready = s_ready;
+      y_pre = false;
     }
-    __syncthreads();
-    if (s_abort) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    for (int e = tid; e < kNB * MC; e += 256) {
-      const int k = e / MC, c = e % MC;
-      Ys[k * SC + c] = (k < bj && c < mc) ? __builtin_nontemporal_load(&B[(int64_t)(j0 + k) * ldb + c0 + c]) : T(0);
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int k = 0; k < kNB; ++k) {
-      const T a = As[r * S + k];
+    if (y_pre) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) acc[e] -= a * Ys[k * SC + cc + e];
+      for (int g = 0; g < 4; ++g) y_cur[g] = y_nxt[g];
+    } else {
+      loady(st, y_cur);
     }
+    T* ys = Ys[st & 1];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) ys[tid + 256 * g] = y_cur[g];
     __syncthreads();
+    if (st + 1 < p) fetch(st + 1, a_nxt);
+    y_pre = st + 1 < ready;
+    if (y_pre) loady(st + 1, y_nxt);
+    const T* yk = ys + KW * wv * MC;
+#pragma unroll
+    for (int kk = 0; kk < KW; ++kk) {
+      const T a = a_cur[kk];
+#pragma unroll
+      for (int c = 0; c < MC; ++c) acc[c] += a * yk[kk * MC + c];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KW; ++kk) a_cur[kk] = a_nxt[kk];
   }
-  // X_w = inv(op(A)_ww) acc
+
+  // reduce the wave partials: Ys[0] = B_w - sum (rows r < bw, columns c < mc)
+  __syncthreads();
+  if (wv > 0) {
 #pragma unroll
-  for (int e = 0; e < 4; ++e) Ys[r * SC + cc + e] = acc[e];
-  const T* D = Dinv + (int64_t)w * kNB * kNB;
-  for (int e = tid; e < kNB * kNB; e += 256) {
-    const int i = e / kNB, jj = e % kNB;
-    As[(dinv_t ? jj : i) * S + (dinv_t ? i : jj)] = D[e];
+    for (int c = 0; c < MC; ++c) Part[wv - 1][r * MC + c] = acc[c];
   }
   __syncthreads();
+  if (wv == 0) {
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+      const T bwv = (r < bw && c < mc) ? B[(int64_t)(w0 + r) * ldb + c0 + c] : T(0);
+      const T sum = acc[c] + Part[0][r * MC + c] + Part[1][r * MC + c] + Part[2][r * MC + c];
+      Ys[0][r * MC + c] = (r < bw && c < mc) ? bwv - sum : T(0);
+    }
+  }
+  __syncthreads();
+  // X_w = inv(op(A)_ww) (B_w - sum): lane r, columns 4 wv .. 4 wv + 3
   T x[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll 8
   for (int k = 0; k < kNB; ++k) {
-    const T d = As[r * S + k];
+    const T d = Dl[r * S + k];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) x[e] += d * Ys[k * SC + cc + e];
+    for (int e = 0; e < 4; ++e) x[e] += d * Ys[0][k * MC + 4 * wv + e];
   }
 #pragma unroll
   for (int e = 0; e < 4; ++e)
-    if (r < bw && cc + e < mc) B[(int64_t)(w0 + r) * ldb + c0 + cc + e] = x[e];
+    if (r < bw && 4 * wv + e < mc) B[(int64_t)(w0 + r) * ldb + c0 + 4 * wv + e] = x[e];
   __threadfence();
   __syncthreads();
   if (tid == 0) __hip_atomic_store(&flags[w * nq + q], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -296,8 +356,13 @@ hipError_t launch_trsm_persistent(int n, int m, const T* A, int64_t lda, bool tr
   const int nb = (n + kNB - 1) / kNB, nq = (m + kSolveChunk - 1) / kSolveChunk;
   hipError_t e = launch_fill_i32(ticket, 0, 1, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((trsm_persistent_kernel<T>), dim3(nb * nq), dim3(256), 0, s, n, m, A, lda, trans, lower_op,
-                     Dinv, dinv_t, B, ldb, ticket, flags, epoch, err, spin_limit);
+  const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && ((lda * sizeof(T)) % 16 == 0);
+  if (vec)
+    hipLaunchKernelGGL((trsm_persistent_kernel<T, true>), dim3(nb * nq), dim3(256), 0, s, n, m, A, lda, trans,
+                       lower_op, Dinv, dinv_t, B, ldb, ticket, flags, epoch, err, spin_limit);
+  else
+    hipLaunchKernelGGL((trsm_persistent_kernel<T, false>), dim3(nb * nq), dim3(256), 0, s, n, m, A, lda, trans,
+                       lower_op, Dinv, dinv_t, B, ldb, ticket, flags, epoch, err, spin_limit);
   return hipGetLastError();
 }
 
