@@ -36,24 +36,36 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n, m, seed):
-    """The oracle (reference algorithm, C, one thread) on one solvec of size n."""
+def cpu_baseline(sizes, m, seed, big_n):
+    """The oracle (the reference algorithm restated in C, one thread) timed on
+    one solvec per size; the largest size is the reported sample, and a c n^3
+    fit from it extrapolates the full-size solves (labelled as such)."""
     import numpy as np
     from oracle import oracle as orc
     rng = np.random.default_rng(seed)
-    b = rng.standard_normal((n, n))
-    a = b @ b.T / n + np.eye(n)
-    rhs = rng.standard_normal((n, m))
-    times = []
-    for _ in range(2):
-        work = np.array(a)
-        t0 = time.perf_counter()
-        orc.solvec(work, rhs)
-        times.append(time.perf_counter() - t0)
-    t = min(times)
+    per = {}
+    for n in sizes:
+        b = rng.standard_normal((n, n))
+        a = b @ b.T / n + np.eye(n)
+        rhs = rng.standard_normal((n, m))
+        times = []
+        for _ in range(2 if n <= 2048 else 1):
+            work = np.array(a)
+            t0 = time.perf_counter()
+            orc.solvec(work, rhs)
+            times.append(time.perf_counter() - t0)
+        per[n] = min(times)
+        log(f"bench: CPU oracle n={n}: {per[n]:.3f} s")
+    n = max(sizes)
+    t = per[n]
+    c = t / solve_flops(n, m)
+    extrap = {str(k): round(c * solve_flops(k, mm), 1) for k, mm in big_n}
     return {"value": round(solve_flops(n, m) / t / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "port",
-            "sample": f"oracle solvec (reference algorithm in C, 1 thread), n={n}, {m} RHS, "
-                      f"A=B*B^T/n+I, best of 2: {t:.2f} s"}
+            "sample": f"oracle solvec (reference algorithm in C, 1 thread), A=B*B^T/n+I, {m} RHS; sample n={n}: "
+                      f"{t:.2f} s",
+            "seconds_by_n": {str(k): round(v, 4) for k, v in per.items()},
+            "extrapolated_seconds_by_n": extrap,
+            "extrapolation": "c*flops fit to the largest sample (not measured)"}
 
 
 def batched_leg(dev, torch, args):
@@ -333,13 +345,15 @@ def main():
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--lam", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=20241202)
-    ap.add_argument("--cpu-n", type=int, default=2048)
+    ap.add_argument("--cpu-sizes", default="512,1024,2048,4096")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--batch", type=int, default=20000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--rocprof-summary", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="summarise rocprofv3 runs of this benchmark into profiles/ and exit")
     ap.add_argument("--rocprof-command", default="")
@@ -426,38 +440,113 @@ def main():
         log(f"bench: residual {check['residual']:.3e} backward error {check['backward_error']:.3e}")
         del l, res, al
 
-    # FP32-MFMA + f64 iterative refinement on the same A (A is not modified).
+    # FP32-MFMA + f64 iterative refinement on the same A (A is not modified),
+    # then on A + (lambda - 1) I for smaller lambda (harder conditioning).
     mixed = None
+    sweep = []
     if not args.no_mixed:
         x32 = torch.zeros(n, m, dtype=torch.float64, device="cuda")
-        log("bench: FP32+IR warmup")
-        info = dev.solve_mixed(a0, rhs0, x32, prec="f32", fallback=True)
-        torch.cuda.synchronize()
-        dev.set_profiling(True)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            info = dev.solve_mixed(a0, rhs0, x32, prec="f32", fallback=True)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        stats32 = dev.kernel_stats()
-        dev.set_profiling(False)
-        ms32 = (t1 - t0) * 1e3 / args.steps
-        g32 = solve_flops(n, m) / (ms32 / 1e3) / 1e9
-        cnt32, tot32, fl32, _ = stats32["trailing_syrk"]
         x64 = copies[-1][1]
-        mixed = {"value": round(g32, 2), "unit": "GFLOP/s", "ms_per_step": round(ms32, 3),
-                 "speedup_vs_f64": round(g32 / gflops, 3), "refinement": info,
-                 "solution_diff_vs_f64_gpu": float((x32 - x64).norm() / x64.norm()),
-                 "syrk_f32_tflops": round(fl32 / (tot32 / 1e3) / 1e12, 2) if tot32 > 0 else None,
-                 "syrk_f32_roofline_frac": round(fl32 / (tot32 / 1e3) / 1e12 / PEAK_TFLOPS["f32"], 4)
-                 if tot32 > 0 else None,
-                 "kernels": {k: round(v[1] / args.steps, 3) for k, v in stats32.items() if v[0]}}
-        log(f"bench: FP32+IR {ms32:.2f} ms/step, {g32:.1f} GFLOP/s, x{g32 / gflops:.2f} vs f64, {info}")
+        lam_cur = args.lam
+        for lam in [args.lam] + [v for v in (1e-2, 1e-4, 1e-6) if v != args.lam]:
+            a0.diagonal().add_(lam - lam_cur)
+            lam_cur = lam
+            log(f"bench: FP32+IR lambda={lam:g} warmup")
+            info = dev.solve_mixed(a0, rhs0, x32, prec="f32", fallback=True)
+            torch.cuda.synchronize()
+            dev.set_profiling(True)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                info = dev.solve_mixed(a0, rhs0, x32, prec="f32", fallback=True)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            stats32 = dev.kernel_stats()
+            dev.set_profiling(False)
+            ms32 = (t1 - t0) * 1e3 / args.steps
+            g32 = solve_flops(n, m) / (ms32 / 1e3) / 1e9
+            res = rhs0 - a0 @ x32
+            bres = float(res.norm() / (torch.linalg.matrix_norm(a0, ord=1) * x32.norm()))
+            del res
+            row = {"lambda": lam, "cond_estimate": round((4.0 + lam) / lam, 1), "ms_per_step": round(ms32, 3),
+                   "value": round(g32, 2), "unit": "GFLOP/s", "speedup_vs_f64": round(g32 / gflops, 3),
+                   "refinement": info, "backward_residual": bres}
+            if lam == args.lam:
+                cnt32, tot32, fl32, _ = stats32["trailing_syrk"]
+                row["solution_diff_vs_f64_gpu"] = float((x32 - x64).norm() / x64.norm())
+                row["syrk_f32_tflops"] = round(fl32 / (tot32 / 1e3) / 1e12, 2) if tot32 > 0 else None
+                row["syrk_f32_roofline_frac"] = (round(fl32 / (tot32 / 1e3) / 1e12 / PEAK_TFLOPS["f32"], 4)
+                                                 if tot32 > 0 else None)
+                row["kernels"] = {k: round(v[1] / args.steps, 3) for k, v in stats32.items() if v[0]}
+                mixed = row
+            sweep.append(row)
+            log(f"bench: FP32+IR lambda={lam:g} {ms32:.2f} ms/step, {g32:.1f} GFLOP/s, x{g32 / gflops:.2f} vs f64, "
+                f"{info}, backward residual {bres:.2e}")
+        a0.diagonal().add_(args.lam - lam_cur)
         del x32
+
+    # C2: n = 16384, 64 right-hand sides, lambda = 0.1, f64
+    c2 = None
+    if not args.no_c2:
+        n2, m2, lam2 = 16384, 64, 0.1
+        del copies
+        g2 = torch.Generator(device="cuda").manual_seed(args.seed + 2)
+        bm2 = torch.randn(n2, n2, dtype=torch.float64, device="cuda", generator=g2)
+        a2 = torch.matmul(bm2, bm2.T).div_(n2)
+        del bm2
+        a2.diagonal().add_(lam2)
+        r2 = torch.randn(n2, m2, dtype=torch.float64, device="cuda", generator=g2)
+        cps = [(a2.clone(), r2.clone()) for _ in range(args.warmup + args.steps)]
+        torch.cuda.synchronize()
+        for i in range(args.warmup):
+            dev.solvec(*cps[i])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            dev.solvec(*cps[args.warmup + i])
+        torch.cuda.synchronize()
+        ms2 = (time.perf_counter() - t0) * 1e3 / args.steps
+        xf = cps[-1][1]
+        res = r2 - a2 @ xf
+        c2 = {"workload": f"spd_solve n={n2} nrhs={m2} lambda={lam2} f64 (llh_dev_solvec_d)",
+              "ms_per_step": round(ms2, 3), "value": round(solve_flops(n2, m2) / (ms2 / 1e3) / 1e9, 2),
+              "unit": "GFLOP/s", "roofline_frac_f64": round(solve_flops(n2, m2) / (ms2 / 1e3) / 1e12 /
+                                                           PEAK_TFLOPS["f64"], 4),
+              "residual": float(res.norm() / (a2.norm() * xf.norm()))}
+        log(f"bench: C2 {c2}")
+        del cps, a2, r2, res
+        torch.cuda.empty_cache()
+
+    # C1: n = 512, 1 right-hand side: latency of one solvec, plain (host arrays
+    # through the boundary) and device-resident
+    c1 = None
+    if not args.no_c1:
+        import numpy as np
+        import linfa_linalg_hip as llh
+        rng1 = np.random.default_rng(args.seed + 1)
+        b1 = rng1.uniform(-1, 1, (512, 512))
+        a1 = b1 @ b1.T / 512 + np.eye(512)
+        v1 = rng1.standard_normal((512, 1))
+        lat_plain, lat_dev = [], []
+        ta = torch.from_numpy(a1).cuda()
+        tb = torch.from_numpy(v1).cuda()
+        for rep in range(25):
+            w1 = np.array(a1)
+            t0 = time.perf_counter()
+            llh.solvec(w1, v1)
+            lat_plain.append(time.perf_counter() - t0)
+            da, db = ta.clone(), tb.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dev.solvec(da, db)
+            torch.cuda.synchronize()
+            lat_dev.append(time.perf_counter() - t0)
+        c1 = {"workload": "spd_solve n=512 nrhs=1 f64", "plain_solvec_p50_us": round(
+            float(np.percentile(lat_plain[5:], 50)) * 1e6, 1), "dev_solvec_p50_us": round(
+            float(np.percentile(lat_dev[5:], 50)) * 1e6, 1)}
+        log(f"bench: C1 {c1}")
 
     batched = None
     if not args.no_batch:
-        del copies
         torch.cuda.empty_cache()
         batched = batched_leg(dev, torch, args)
         log(f"bench: batched {batched['solves_per_s']} solves/s ({batched['ms']} ms), "
@@ -469,8 +558,8 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log(f"bench: CPU baseline sample n={args.cpu_n}")
-        cpu = cpu_baseline(args.cpu_n, m, args.seed)
+        sizes = [int(v) for v in args.cpu_sizes.split(",")]
+        cpu = cpu_baseline(sizes, m, args.seed, [(16384, 64), (n, m)])
 
     out = {
         "metric": "SPD solve GFLOP/s at n=32768 (f64 POTRF + forward/back solve, 16 RHS)",
@@ -491,6 +580,9 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp32_ir": mixed,
+        "fp32_ir_lambda_sweep": sweep,
+        "c2": c2,
+        "c1": c1,
         "batched": batched,
         "serving": serving,
         "kernels": kernels,

@@ -2,20 +2,36 @@
 // (the trailing updates there round differently from the reference anyway, so
 // the block may too; small whole matrices keep the exact-order kernel).
 //
-// The block is staged in LDS.  It is factored in eight panels of 8 columns:
-// wave 0 factors a panel with each lane holding its row's 8 panel values in
-// registers, the pivot and the column values broadcast by v_readlane (no
-// barrier inside a panel); then all four waves apply the panel's rank-8 update
-// to the rest of the block.  The inverse of the factor is then formed in LDS by
-// the recursive doubling of trtri64_block.  Pivot test, failure row and
-// write-back follow cholesky.rs:54-74: on a failure at row f the block holds
-// rows < f and L[f, ..f].
+// One wave does it all, in panels of 8 columns.  Lane i holds row i of the
+// block in registers.  Inside a panel the column chain uses registers only:
+// the pivot and each L[k, j] come from their lanes by v_readlane.  The panel is
+// then written to a row-major copy of L in LDS, and the rank-8 update of the
+// columns to its right reads L[k, j0 .. j0+8) as four 128-bit uniform-address
+// (broadcast) loads per row k, independent of one another, so they pipeline
+// behind the FMAs.  The inverse runs the same way with lane c holding column c
+// of inv(L): inside a panel x_k is scaled by 1/L[k, k] and eliminated from the
+// panel's later rows; then the rows below take the panel's rank-8 update.
+// Each lane loads and stores its own row, all its loads issued together.  Rows
+// beyond b are identity rows, so the block factors to diag(L, I) and its
+// inverse likewise.  Pivot test, failure row and write-back follow
+// cholesky.rs:54-74: on a failure at row f the block holds rows < f and
+// L[f, ..f].
+#include <type_traits>
+#include <utility>
+
 #include "llh_common.h"
-#include "k_trsm_impl.h"
 #include "llh_kernels.h"
 
 namespace llh {
 namespace {
+
+// Calls f(std::integral_constant<int, I>) for I = 0 .. n-1: the panel loops are
+// expanded at compile time, so every index into the register arrays is a
+// constant (a rolled loop would move them to scratch memory).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for(std::integer_sequence<int, I...>, F&& f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
 
 __device__ __forceinline__ double bcast(double v, int lane) {
   const long long b = __double_as_longlong(v);
@@ -27,131 +43,146 @@ __device__ __forceinline__ float bcast(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// TS: storage type of the block and of the inverse; T: arithmetic type.  Sm,
-// Xs, Ts: LDS arrays of 64 x 65 elements.
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+constexpr int kDiagS = kNB + 2;  // LDS row stride (keeps rows 16-byte aligned)
+constexpr int kDiagP = 8;        // panel width
+
+template <typename T>
+struct DiagLds {
+  T l[kNB * kDiagS];  // L (row-major), written a panel at a time
+  T rinv[kNB];        // 1 / L[j, j]
+};
+
+// TS: storage type of the block and of the inverse; T: arithmetic type.
+// Launched with exactly one wave.
 template <typename TS, typename T>
-__device__ void diag_fast_block(TS* __restrict__ a, int64_t lda, int b, TS* __restrict__ inv,
-                                int32_t* __restrict__ info, int64_t info_base, T* Sm, T* Xs, T* Ts,
-                                int* s_fail_p) {
-  constexpr int N = kNB, S = kNB + 1, P = 8;
-  int& s_fail = *s_fail_p;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // stage the lower triangle (identity beyond b)
+__device__ __forceinline__ void diag_wave_block(TS* __restrict__ a, int64_t lda, int b, TS* __restrict__ inv,
+                                                int32_t* __restrict__ info, int64_t info_base, DiagLds<T>& sm) {
+  constexpr int N = kNB, S = kDiagS, P = kDiagP;
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const int i = threadIdx.x;  // this lane's row (factor) / column (loads, stores, inverse)
+  T* l = sm.l;  // row-major L, published a panel at a time
+  // lane i loads its own row; masked entries read the block's first element
+  // (always in bounds) so that the 64 loads carry no branches and issue together
+  T r[N];
   {
-    TS v[16];
+    const TS* row = a + (int64_t)min(i, b - 1) * lda;
 #pragma unroll
-    for (int g = 0; g < 16; ++g) {
-      const int e = tid + 256 * g, i = e / N, j = e % N;
-      v[g] = (i < b && j <= i) ? a[(int64_t)i * lda + j] : TS(0);
-    }
-#pragma unroll
-    for (int g = 0; g < 16; ++g) {
-      const int e = tid + 256 * g, i = e / N, j = e % N;
-      Sm[i * S + j] = (i < b) ? T(v[g]) : (i == j ? T(1) : T(0));
+    for (int j = 0; j < N; ++j) {
+      const bool in = i < b && j <= i;
+      const T v = T(*(in ? row + j : a));
+      r[j] = in ? v : ((i >= b && i == j) ? T(1) : T(0));
     }
   }
-  if (tid == 0) s_fail = -1;
-  __syncthreads();
+  // (no early exit: a failure only turns the remaining steps off, so every loop
+  // keeps its constant trip count and r stays in registers)
   int fail = -1;
-  for (int c0 = 0; c0 < b; c0 += P) {
-    if (wave == 0) {
-      T v[P];
+  static_for(std::make_integer_sequence<int, N / P>{}, [&](auto pi) {
+    constexpr int j0 = decltype(pi)::value * P;
 #pragma unroll
-      for (int c = 0; c < P; ++c) v[c] = Sm[lane * S + c0 + c];
-      int f = -1;
+    for (int c = 0; c < P; ++c) {
+      const int j = j0 + c;
+      if (fail < 0) {
+        const T d = bcast(r[j], j);
+        if (d <= T(0)) {  // cholesky.rs:69 (NaN passes)
+          fail = j;
+        } else {
+          const T piv = sqrt_rn(d);
+          const T rp = T(1) / piv;
+          r[j] = (i == j) ? piv : r[j] * rp;
+          if (i == 0) sm.rinv[j] = rp;
 #pragma unroll
-      for (int c = 0; c < P; ++c) {
-        const int col = c0 + c;
-        if (col < b && f < 0) {
-          const T d = bcast(v[c], col);
-          if (d <= T(0)) {
-            f = col;  // cholesky.rs:69 (NaN passes)
-          } else {
-            const T r = sqrt_rn(d);
-            v[c] = lane > col ? v[c] / r : (lane == col ? r : v[c]);
-#pragma unroll
-            for (int q = c + 1; q < P; ++q) {
-              const T lq = bcast(v[c], c0 + q);
-              if (lane >= c0 + q) v[q] -= v[c] * lq;
-            }
-          }
+          for (int q = c + 1; q < P; ++q) r[j0 + q] -= r[j] * bcast(r[j], j0 + q);
         }
       }
-      // write the panel: every finished column, and on failure the columns
-      // before the failing one (which hold row f's L[f, ..f])
-#pragma unroll
-      for (int c = 0; c < P; ++c)
-        if (lane >= c0 && (f < 0 || c0 + c < f)) Sm[lane * S + c0 + c] = v[c];
-      if (lane == 0) s_fail = f;
     }
-    __syncthreads();
-    fail = s_fail;
-    if (fail >= 0) break;
-    // rank-8 update of the rest: thread (row j = tid / 4, quarter q) updates
-    // S[j][k] for k = c0 + P + q + 4t <= j.
-    const int j = tid >> 2, q = tid & 3, k1 = c0 + P;
-    if (j >= k1 && j < b) {
-      T lj[P];
+    if (fail < 0) {
+      // publish the panel (this lane's row), then the rank-8 update to its right
 #pragma unroll
-      for (int p = 0; p < P; ++p) lj[p] = Sm[j * S + c0 + p];
-      for (int k = k1 + q; k <= j; k += 4) {
-        T s = Sm[j * S + k];
+      for (int q = 0; q < P; q += 2) *reinterpret_cast<v2*>(&l[i * S + j0 + q]) = v2{r[j0 + q], r[j0 + q + 1]};
+      __builtin_amdgcn_wave_barrier();
+      // (panel column pairs outermost: the pair stays in registers while the
+      // rows stream past, which keeps the register count bounded)
 #pragma unroll
-        for (int p = 0; p < P; ++p) s -= lj[p] * Sm[k * S + c0 + p];
-        Sm[j * S + k] = s;
+      for (int q = 0; q < P; q += 2) {
+        const T p0 = r[j0 + q], p1 = r[j0 + q + 1];
+#pragma unroll
+        for (int k = j0 + P; k < N; ++k) {
+          const v2 lk = *reinterpret_cast<const v2*>(&l[k * S + j0 + q]);
+          r[k] = fma_t(-p0, lk[0], fma_t(-p1, lk[1], r[k]));
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
-    __syncthreads();
-  }
-  // write back: rows < f (or all rows) with their lower part, row f up to f-1
+  });
+  // write back from registers: rows < f (or all rows) with their lower part,
+  // row f up to f-1
   {
-    const int rows = fail < 0 ? b : fail + 1;
-    for (int e = tid; e < N * N; e += 256) {
-      const int i = e / N, j = e % N;
-      if (i < rows && (i == fail ? j < i : j <= i)) a[(int64_t)i * lda + j] = TS(Sm[i * S + j]);
-    }
+    const int last = (fail < 0 || i < fail) ? i : (i == fail ? i - 1 : -1);
+    TS* row = a + (int64_t)i * lda;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if (i < b && j <= last) row[j] = TS(r[j]);
   }
   if (fail >= 0) {
-    if (tid == 0) atomicMin(info, (int32_t)(info_base + fail));
+    if (i == 0) atomicMin(info, (int32_t)(info_base + fail));
     return;
   }
-  // inverse of the factor held in Sm (read through a generic pointer, ld S;
-  // Sm doubles as the inverse's copy of the block)
-  __syncthreads();
-  trtri64_block<T, TS, T>(b, Sm, S, false, false, inv, Sm, Xs, Ts);
+  // inv(L): lane c solves L x = e_c in panels of 8
+  const int c = i;
+  T x[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) x[k] = (k == c) ? T(1) : T(0);
+  static_for(std::make_integer_sequence<int, N / P>{}, [&](auto pi) {
+    constexpr int k0 = decltype(pi)::value * P;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const int k = k0 + q;
+      x[k] *= sm.rinv[k];
+#pragma unroll
+      for (int q2 = q + 1; q2 < P; ++q2) x[k0 + q2] -= l[(k0 + q2) * S + k] * x[k];
+    }
+#pragma unroll
+    for (int q = 0; q < P; q += 2) {
+      const T p0 = x[k0 + q], p1 = x[k0 + q + 1];
+#pragma unroll
+      for (int ii = k0 + P; ii < N; ++ii) {
+        const v2 li = *reinterpret_cast<const v2*>(&l[ii * S + k0 + q]);
+        x[ii] = fma_t(-li[0], p0, fma_t(-li[1], p1, x[ii]));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  });
+#pragma unroll
+  for (int ii = 0; ii < N; ++ii) inv[ii * N + c] = TS(x[ii]);
 }
 
 template <typename TS, typename T>
-__global__ __launch_bounds__(256) void diag_fast_kernel(TS* __restrict__ a, int64_t lda, int b,
-                                                        TS* __restrict__ inv, int32_t* __restrict__ info,
-                                                        int64_t info_base, const int32_t* __restrict__ gate) {
+__global__ __launch_bounds__(64) void diag_fast_kernel(TS* __restrict__ a, int64_t lda, int b,
+                                                       TS* __restrict__ inv, int32_t* __restrict__ info,
+                                                       int64_t info_base, const int32_t* __restrict__ gate) {
   if (gate != nullptr && *gate != kNoFail) return;
-  __shared__ T Sm[kNB * (kNB + 1)];  // the block, then (aliased) the inverse's working copy
-  __shared__ T Xs[kNB * (kNB + 1)];
-  __shared__ T Ts[kNB * (kNB + 1)];
-  __shared__ int s_fail;
-  diag_fast_block<TS, T>(a, lda, b, inv, info, info_base, Sm, Xs, Ts, &s_fail);
+  __shared__ __attribute__((aligned(16))) DiagLds<T> sm;
+  diag_wave_block<TS, T>(a, lda, b, inv, info, info_base, sm);
 }
 
 // Grouped form for the batch planner: work w = (item, k0) factors and inverts
 // the block at (k0, k0) of its system; a failure sets the system's status.
 template <typename T>
-__global__ __launch_bounds__(256) void grp_diag_fast_kernel(const BatchBase<T>* __restrict__ base,
-                                                            const BatchItem* __restrict__ items,
-                                                            const StepWork* __restrict__ work, int count) {
+__global__ __launch_bounds__(64) void grp_diag_fast_kernel(const BatchBase<T>* __restrict__ base,
+                                                           const BatchItem* __restrict__ items,
+                                                           const StepWork* __restrict__ work, int count) {
   if ((int)blockIdx.x >= count) return;
   const StepWork w = work[blockIdx.x];
   const BatchItem it = items[w.item];
   const BatchBase<T> bb = *base;
   int32_t* st = bb.status + it.id;
   if (*st != kNoFail) return;
-  __shared__ T Sm[kNB * (kNB + 1)];
-  __shared__ T Xs[kNB * (kNB + 1)];
-  __shared__ T Ts[kNB * (kNB + 1)];
-  __shared__ int s_fail;
-  diag_fast_block<T, T>(bb.a + it.a_off + (int64_t)w.k0 * it.lda + w.k0, it.lda, min(kNB, it.n - w.k0),
-                        bb.linv + it.linv_off + (int64_t)(w.k0 / kNB) * kNB * kNB, st, w.k0, Sm, Xs, Ts,
-                        &s_fail);
+  __shared__ __attribute__((aligned(16))) DiagLds<T> sm;
+  diag_wave_block<T, T>(bb.a + it.a_off + (int64_t)w.k0 * it.lda + w.k0, it.lda, min(kNB, it.n - w.k0),
+                        bb.linv + it.linv_off + (int64_t)(w.k0 / kNB) * kNB * kNB, st, w.k0, sm);
 }
 
 }  // namespace
@@ -160,7 +191,7 @@ template <typename T>
 hipError_t launch_grp_diag_fast(const BatchBase<T>* base, const BatchItem* items, const StepWork* work,
                                 int count, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL((grp_diag_fast_kernel<T>), dim3(count), dim3(256), 0, s, base, items, work, count);
+  hipLaunchKernelGGL((grp_diag_fast_kernel<T>), dim3(count), dim3(64), 0, s, base, items, work, count);
   return hipGetLastError();
 }
 template hipError_t launch_grp_diag_fast<double>(const BatchBase<double>*, const BatchItem*, const StepWork*,
@@ -173,7 +204,7 @@ hipError_t launch_diag_fast(TS* a, int64_t lda, int b, TS* inv, int32_t* info, i
                             const int32_t* gate, hipStream_t s) {
   if (b <= 0) return hipSuccess;
   if (b > kNB) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((diag_fast_kernel<TS, T>), dim3(1), dim3(256), 0, s, a, lda, b, inv, info, info_base, gate);
+  hipLaunchKernelGGL((diag_fast_kernel<TS, T>), dim3(1), dim3(64), 0, s, a, lda, b, inv, info, info_base, gate);
   return hipGetLastError();
 }
 

@@ -212,9 +212,9 @@ __device__ inline void tri_tile(int lin, int tiles_m, int tiles_n, int R, int& t
 // SQ: the square triangular case (TRI and M == N), i.e. the trailing SYRK.
 template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool SQ, bool VEC>
 __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_kernel(int M, int N, int K, T alpha,
-                                                                 const T* __restrict__ A, int64_t lda,
+                                                                 const T* A, int64_t lda,
                                                                  const T* __restrict__ B, int64_t ldb, T beta,
-                                                                 T* __restrict__ C, int64_t ldc,
+                                                                 T* C, int64_t ldc,
                                                                  const int32_t* __restrict__ gate, int tiles_m,
                                                                  int tiles_n, int total) {
   static_assert(BM % BN == 0, "tile rows are a multiple of tile columns");
@@ -298,11 +298,19 @@ hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha,
                        hipStream_t s) {
   if (M <= 0 || N <= 0) return hipSuccess;
   if (K <= 0) K = 0;
-  if (!ta && tb && N > 64) {
+  if (!ta && tb) {
     // Large updates: 256 x 128 tiles on 8 waves (2 per SIMD) cut the operand
     // traffic per flop by a quarter; smaller ones keep 128 x 128 tiles so the
-    // grid still fills the 256 CUs.  Triangular updates enumerate the full
-    // rectangle; tiles wholly above the diagonal exit at once.
+    // grid still fills the 256 CUs; narrow ones (N <= 64, the panel solve) and
+    // small ones that would give fewer than 128 such tiles use 64 x 64 tiles.
+    // Triangular updates enumerate the full rectangle; tiles wholly above the
+    // diagonal exit at once.  A and C may alias when every tile holds all of
+    // its rows' k (the in-place panel solve: N = K <= 64, one tile column).
+    const int64_t t128 = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
+    if (N <= 64 || t128 < 128) {
+      if (tri) return launch_nt<T, 64, 64, 2, 2, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
+      return launch_nt<T, 64, 64, 2, 2, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
+    }
     const bool big = (int64_t)M * N >= (int64_t)4096 * 4096;
     if (tri) {
       if (big) return launch_nt<T, 256, 128, 4, 2, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
