@@ -129,6 +129,15 @@ llh_status llh_context_kernel_stats(llh_context* ctx, int kernel_class, llh_kern
       llh_context* ctx, size_t rows, size_t cols, const T* a, ptrdiff_t rs, ptrdiff_t cs, \
       size_t b_rows, size_t b_cols, T* b, ptrdiff_t brs, ptrdiff_t bcs, llh_uplo uplo,   \
       llh_error* err);                                                                    \
+  /* solve_triangular_system(a, b, uplo, diag_fn), triangular.rs:133-144 (crate-      \
+     private; kernel :95-128) with diag_fn(i) = |diag[i * dstride]| when abs_diag, else  \
+     diag[i * dstride]: the pivots come from a separate vector, as the QR solves pass    \
+     |R_ii| (qr.rs:145-150, 172-177).  Reads only the strict `uplo` triangle of a (its   \
+     diagonal is never read) and the n entries of diag; errors as the solve above. */    \
+  llh_status llh_solve_triangular_system_##sfx(                                           \
+      llh_context* ctx, size_t rows, size_t cols, const T* a, ptrdiff_t rs, ptrdiff_t cs, \
+      const T* diag, ptrdiff_t dstride, int abs_diag, size_t b_rows, size_t b_cols, T* b, \
+      ptrdiff_t brs, ptrdiff_t bcs, llh_uplo uplo, llh_error* err);                       \
   /* SolveCInplace::solvec_inplace, cholesky.rs:122 (impl :136-144).  A is left         \
      holding the dirty factor, also when the later WrongRows check fails. */             \
   llh_status llh_solvec_inplace_##sfx(llh_context* ctx, size_t rows, size_t cols, T* a,    \
@@ -163,7 +172,20 @@ LLH_PLAIN_DECL(float, s)
                                             T* b, size_t ldb, llh_error* err);             \
   /* solvec_inplace on device memory (cholesky.rs:136-144): A := dirty factor, B := X. */ \
   llh_status llh_dev_solvec_##sfx(llh_context* ctx, size_t n, T* a, size_t lda, size_t m,  \
-                                  T* b, size_t ldb, llh_error* err);
+                                  T* b, size_t ldb, llh_error* err);                      \
+  /* LOBPCG's orthonormalize(v), lobpcg/algorithm.rs:75-97 (private to the LOBPCG        \
+     module, the Cholesky path's own consumer): G = V^T V, L = cholesky_into(G), then     \
+     V := (L^-1 V^T)^T.  v: n x k row-major on the device (ldv >= k), overwritten with    \
+     U; r: k x k (ldr >= k) receives L with a zero upper triangle.  NotPositiveDefinite   \
+     (column in err) when V^T V is not positive definite; v is then unchanged. */          \
+  llh_status llh_dev_orthonormalize_##sfx(llh_context* ctx, size_t n, size_t k, T* v,      \
+                                          size_t ldv, T* r, size_t ldr, llh_error* err);   \
+  /* LOBPCG's apply_constraints(v, cholesky_yy, y), lobpcg/algorithm.rs:60-73:            \
+     V := V - Y (L_yy^-1 (Y^T V)), with y n x ky (ldy >= ky) and l_yy the ky x ky lower    \
+     Cholesky factor of Y^T Y (only its lower triangle is read). */                        \
+  llh_status llh_dev_apply_constraints_##sfx(llh_context* ctx, size_t n, size_t k, T* v,   \
+                                             size_t ldv, size_t ky, const T* y, size_t ldy, \
+                                             const T* l_yy, size_t ldl, llh_error* err);
 
 LLH_DEV_DECL(double, d)
 LLH_DEV_DECL(float, s)

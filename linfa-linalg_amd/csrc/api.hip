@@ -300,7 +300,8 @@ llh_status is_triangular_plain(llh_context* ctx, size_t rows, size_t cols, const
 template <typename T>
 llh_status solve_triangular_plain(llh_context* ctx, size_t rows, size_t cols, const T* a, ptrdiff_t rs,
                                   ptrdiff_t cs, size_t b_rows, size_t b_cols, T* b, ptrdiff_t brs,
-                                  ptrdiff_t bcs, llh_uplo uplo, llh_error* err) {
+                                  ptrdiff_t bcs, llh_uplo uplo, llh_error* err, const T* diag = nullptr,
+                                  ptrdiff_t dstride = 0, int abs_diag = 0) {
   if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
   if (rows != cols) return not_square(err, rows, cols);  // triangular.rs:102
   const size_t n = rows;
@@ -310,6 +311,14 @@ llh_status solve_triangular_plain(llh_context* ctx, size_t rows, size_t cols, co
   std::vector<T> ha(n * n), hb(n * b_cols);
   gather(a, rs, cs, n, n, ha.data());
   gather(b, brs, bcs, n, b_cols, hb.data());
+  if (diag != nullptr) {
+    // diag_fn: the kernel (triangular.rs:113-124) reads a's diagonal only as the
+    // pivot, so the packed copy carries the caller's pivots in its place
+    for (size_t i = 0; i < n; ++i) {
+      const T d = diag[(ptrdiff_t)i * dstride];
+      ha[i * n + i] = (abs_diag && d < T(0)) ? -d : d;
+    }
+  }
   void *dav = nullptr, *dbv = nullptr;
   CHECK_HIP(ctx_buffer(ctx, 0, n * n * sizeof(T), &dav), "device buffer");
   CHECK_HIP(ctx_buffer(ctx, 1, n * b_cols * sizeof(T), &dbv), "device buffer");
@@ -505,6 +514,111 @@ llh_status dev_solvec(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T*
   return ok(err);
 }
 
+// ---- LOBPCG consumers (lobpcg/algorithm.rs:60-97) --------------------------
+
+// Scratch of the LOBPCG forms in context slot 3: the Gram partial tiles, then a
+// k x k (or ky x k) matrix, then (k > 64 only) an n x k product.
+template <typename T>
+hipError_t lobpcg_scratch(llh_context* ctx, int n, int ka, int kb, size_t extra, T** partial, T** small,
+                          T** big) {
+  int chunks = 0, rpc = 0;
+  const size_t np = (size_t)gram_partial_elems(n, ka, kb, &chunks, &rpc);
+  const size_t ns = (size_t)ka * kb;
+  void* p = nullptr;
+  hipError_t e = ctx_buffer(ctx, 3, (np + ns + extra) * sizeof(T), &p);
+  if (e != hipSuccess) return e;
+  *partial = static_cast<T*>(p);
+  *small = *partial + np;
+  *big = *small + ns;
+  return hipSuccess;
+}
+
+// orthonormalize(v), lobpcg/algorithm.rs:75-97: G = V^T V; L = cholesky_into(G)
+// (the clean lower factor, into r); U = (L^-1 V^T)^T = V L^-T, into v.  The
+// triangular solve on V^T is done as V W^T with W = L^-1 (a k x k solve on the
+// identity), one MFMA product over the n rows.
+template <typename T>
+llh_status dev_orthonormalize(llh_context* ctx, size_t n, size_t k, T* v, size_t ldv, T* r, size_t ldr,
+                              llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (k == 0) return ok(err);
+  if (v == nullptr || r == nullptr || ldv < k || ldr < k) return invalid(err, "bad matrix arguments");
+  if (n > 0x7fffffff || k > 0x7fffffff) return invalid(err, "matrix too large");
+  const int N = (int)n, K = (int)k;
+  T *part = nullptr, *w = nullptr, *tmp = nullptr;
+  CHECK_HIP(lobpcg_scratch<T>(ctx, N, K, K, K > 64 ? n * k : 0, &part, &w, &tmp), "scratch");
+  hipStream_t s = ctx->stream;
+  CHECK_HIP(launch_gram<T>(N, K, K, v, (int64_t)ldv, v, (int64_t)ldv, r, (int64_t)ldr, part, s), "gram");
+  void* dv = nullptr;
+  CHECK_HIP(ctx_dinv(ctx, K, sizeof(T), &dv), "inverse buffer");
+  T* dinv = static_cast<T*>(dv);
+  CHECK_HIP(potrf_enqueue<T>(r, (int64_t)ldr, K, ctx->d_info, true, dinv, s, ctx->profiler(), false,
+                            ctx_lookahead(ctx)),
+            "potrf");
+  CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, s), "status");
+  CHECK_HIP(hipStreamSynchronize(s), "synchronize");
+  if (ctx->h_info[0] != kNoFail) return not_pd(err, ctx->h_info[0]);  // cholesky_into()? (:81)
+  SolveSync sync;
+  CHECK_HIP(ctx_sync(ctx, K, K, &sync), "solve sync");
+  CHECK_HIP(launch_set_identity<T>(w, (int64_t)K, K, s), "identity");
+  CHECK_HIP(trsm_enqueue<T>(false, false, K, r, (int64_t)ldr, K, w, (int64_t)K, dinv, false, nullptr, &sync, s,
+                            ctx->profiler()),
+            "inverse");
+  commit_sync(ctx, sync);
+  if (n > 0) {
+    if (K <= 64) {
+      // one 64-column tile holds all k columns of its rows: in place
+      CHECK_HIP(launch_gemm<T>(false, true, false, N, K, K, T(1), v, (int64_t)ldv, w, (int64_t)K, T(0), v,
+                               (int64_t)ldv, nullptr, s),
+                "product");
+    } else {
+      CHECK_HIP(launch_gemm<T>(false, true, false, N, K, K, T(1), v, (int64_t)ldv, w, (int64_t)K, T(0), tmp,
+                               (int64_t)K, nullptr, s),
+                "product");
+      CHECK_HIP(hipMemcpy2DAsync(v, ldv * sizeof(T), tmp, k * sizeof(T), k * sizeof(T), n, hipMemcpyDeviceToDevice,
+                                 s),
+                "copy");
+    }
+  }
+  CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
+  CHECK_HIP(hipStreamSynchronize(s), "synchronize");
+  if (ctx->h_info[2] != 0) return solve_stalled(err);
+  return ok(err);
+}
+
+// apply_constraints(v, cholesky_yy, y), lobpcg/algorithm.rs:60-73:
+// gram_yv = Y^T V; u = cholesky_yy.solve_triangular_into(gram_yv, Lower);
+// V = V - Y u (general_mat_mul(-1, y, u, 1, v)).
+template <typename T>
+llh_status dev_apply_constraints(llh_context* ctx, size_t n, size_t k, T* v, size_t ldv, size_t ky, const T* y,
+                                 size_t ldy, const T* l_yy, size_t ldl, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (k == 0 || ky == 0 || n == 0) return ok(err);
+  if (v == nullptr || y == nullptr || l_yy == nullptr || ldv < k || ldy < ky || ldl < ky)
+    return invalid(err, "bad matrix arguments");
+  if (n > 0x7fffffff || k > 0x7fffffff || ky > 0x7fffffff) return invalid(err, "matrix too large");
+  const int N = (int)n, K = (int)k, KY = (int)ky;
+  T *part = nullptr, *g = nullptr, *unused = nullptr;
+  CHECK_HIP(lobpcg_scratch<T>(ctx, N, KY, K, 0, &part, &g, &unused), "scratch");
+  hipStream_t s = ctx->stream;
+  CHECK_HIP(launch_gram<T>(N, KY, K, y, (int64_t)ldy, v, (int64_t)ldv, g, (int64_t)K, part, s), "gram");
+  void* dv = nullptr;
+  CHECK_HIP(ctx_dinv(ctx, KY, sizeof(T), &dv), "inverse buffer");
+  SolveSync sync;
+  CHECK_HIP(ctx_sync(ctx, KY, K, &sync), "solve sync");
+  CHECK_HIP(trsm_enqueue<T>(false, false, KY, l_yy, (int64_t)ldl, K, g, (int64_t)K, nullptr, false,
+                            static_cast<T*>(dv), &sync, s, ctx->profiler()),
+            "solve");
+  commit_sync(ctx, sync);
+  CHECK_HIP(launch_gemm<T>(false, false, false, N, K, KY, T(-1), y, (int64_t)ldy, g, (int64_t)K, T(1), v,
+                           (int64_t)ldv, nullptr, s),
+            "update");
+  CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
+  CHECK_HIP(hipStreamSynchronize(s), "synchronize");
+  if (ctx->h_info[2] != 0) return solve_stalled(err);
+  return ok(err);
+}
+
 }  // namespace
 
 extern "C" {
@@ -633,6 +747,15 @@ llh_status llh_context_kernel_stats(llh_context* ctx, int kernel_class, llh_kern
     return solve_triangular_plain<T>(ctx, rows, cols, a, rs, cs, b_rows, b_cols, b, brs, bcs, uplo,     \
                                      err);                                                              \
   }                                                                                                     \
+  llh_status llh_solve_triangular_system_##sfx(                                                         \
+      llh_context* ctx, size_t rows, size_t cols, const T* a, ptrdiff_t rs, ptrdiff_t cs, const T* diag,  \
+      ptrdiff_t dstride, int abs_diag, size_t b_rows, size_t b_cols, T* b, ptrdiff_t brs, ptrdiff_t bcs,  \
+      llh_uplo uplo, llh_error* err) {                                                                  \
+    if (diag == nullptr && rows > 0 && rows == cols && b_rows == rows && b_cols > 0)                    \
+      return invalid(err, "null diagonal pointer");                                                     \
+    return solve_triangular_plain<T>(ctx, rows, cols, a, rs, cs, b_rows, b_cols, b, brs, bcs, uplo, err, \
+                                     diag, dstride, abs_diag);                                          \
+  }                                                                                                     \
   llh_status llh_solvec_inplace_##sfx(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdiff_t rs,    \
                                       ptrdiff_t cs, size_t b_rows, size_t b_cols, T* b, ptrdiff_t brs,  \
                                       ptrdiff_t bcs, llh_error* err) {                                  \
@@ -659,6 +782,15 @@ llh_status llh_context_kernel_stats(llh_context* ctx, int kernel_class, llh_kern
   llh_status llh_dev_solvec_##sfx(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T* b,         \
                                   size_t ldb, llh_error* err) {                                         \
     return dev_solvec<T>(ctx, n, a, lda, m, b, ldb, err);                                               \
+  }                                                                                                     \
+  llh_status llh_dev_orthonormalize_##sfx(llh_context* ctx, size_t n, size_t k, T* v, size_t ldv, T* r,  \
+                                          size_t ldr, llh_error* err) {                                 \
+    return dev_orthonormalize<T>(ctx, n, k, v, ldv, r, ldr, err);                                       \
+  }                                                                                                     \
+  llh_status llh_dev_apply_constraints_##sfx(llh_context* ctx, size_t n, size_t k, T* v, size_t ldv,     \
+                                             size_t ky, const T* y, size_t ldy, const T* l_yy,          \
+                                             size_t ldl, llh_error* err) {                              \
+    return dev_apply_constraints<T>(ctx, n, k, v, ldv, ky, y, ldy, l_yy, ldl, err);                     \
   }
 
 LLH_PLAIN_IMPL(double, d)

@@ -114,6 +114,13 @@ hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha,
                        const T* B, int64_t ldb, T beta, T* C, int64_t ldc, const int32_t* gate,
                        hipStream_t s);
 
+// k_gram.hip: C (ka x kb, ldc) = A^T B for tall row-major A (n x ka) and B
+// (n x kb), split over row chunks; partial holds gram_partial_elems(...) elements.
+int64_t gram_partial_elems(int n, int ka, int kb, int* chunks, int* rows_per_chunk);
+template <typename T>
+hipError_t launch_gram(int n, int ka, int kb, const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc,
+                       T* partial, hipStream_t s);
+
 // k_gemm.hip: C = beta*C + alpha*A*A^T on the lower triangle of the n x n C
 // (A is n x k, k-contiguous rows): the trailing update of the blocked POTRF.
 template <typename T>

@@ -36,7 +36,7 @@ __all__ = [
     "cholesky_inplace_dirty", "cholesky_into_dirty", "cholesky_inplace", "cholesky_into",
     "cholesky_dirty", "cholesky", "solvec_inplace", "solvec_into", "solvec", "invc_inplace",
     "invc", "triangular_inplace", "into_triangular", "is_triangular", "solve_triangular_inplace",
-    "solve_triangular_into", "solve_triangular", "Context", "library_path",
+    "solve_triangular_into", "solve_triangular", "solve_triangular_system", "Context", "library_path",
 ]
 
 
@@ -266,6 +266,22 @@ def solve_triangular_into(a, b, uplo):
 
 def solve_triangular(a, b, uplo):
     return solve_triangular_into(a, np.array(b, copy=True), uplo)
+
+
+def solve_triangular_system(a, b, uplo, diag, abs_diag=True):
+    """solve_triangular_system(a, &mut b, uplo, diag_fn) (triangular.rs:133-144)
+    with diag_fn(i) = |diag[i]| (abs_diag, as the QR solves pass it, qr.rs:145-150)
+    or diag[i]; b is solved in place and returned.  a's diagonal is not read."""
+    a = np.asarray(a)
+    diag = np.asarray(diag)
+    if a.dtype != b.dtype or diag.dtype != a.dtype:
+        raise TypeError("a, b and diag must have the same dtype")
+    if diag.ndim != 1:
+        raise ValueError("diag must be 1-D")
+    ds = diag.strides[0] // diag.itemsize
+    _call("solve_triangular_system", a, *_view(a), ctypes.c_void_p(diag.ctypes.data), ds, int(bool(abs_diag)),
+          *_view(_writable(b)), int(uplo))
+    return b
 
 
 from . import device  # noqa: E402,F401  (device-resident forms over torch tensors)

@@ -130,11 +130,15 @@ def _declare(lib):
         f("is_triangular").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _int, ctypes.POINTER(_int), _err]
         f("solve_triangular_inplace").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _sz, _sz, _vp, _pd, _pd,
                                                   _int, _err]
+        f("solve_triangular_system").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _vp, _pd, _int, _sz, _sz, _vp,
+                                                 _pd, _pd, _int, _err]
         f("solvec_inplace").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _sz, _sz, _vp, _pd, _pd, _err]
         f("invc_inplace").argtypes = [_vp, _sz, _sz, _vp, _pd, _pd, _vp, _pd, _pd, _err]
         f("dev_potrf").argtypes = [_vp, _sz, _vp, _sz, _int, _err]
         f("dev_solve_triangular").argtypes = [_vp, _int, _int, _sz, _vp, _sz, _sz, _vp, _sz, _err]
         f("dev_solvec").argtypes = [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _err]
+        f("dev_orthonormalize").argtypes = [_vp, _sz, _sz, _vp, _sz, _vp, _sz, _err]
+        f("dev_apply_constraints").argtypes = [_vp, _sz, _sz, _vp, _sz, _sz, _vp, _sz, _vp, _sz, _err]
 
 
 def _share_torch_runtime():

@@ -75,6 +75,33 @@ def solvec(a, b):
     return b
 
 
+def orthonormalize(v):
+    """LOBPCG's orthonormalize (lobpcg/algorithm.rs:75-97): returns (u, r) with
+    r = cholesky(V^T V) (lower) and u = V r^-T; u is v, overwritten in place."""
+    import torch
+    _check(v)
+    n, k = v.shape
+    r = torch.zeros(k, k, dtype=v.dtype, device=v.device)
+    _call("orthonormalize", v, n, k, ctypes.c_void_p(v.data_ptr()), v.stride(0), ctypes.c_void_p(r.data_ptr()),
+          r.stride(0))
+    return v, r
+
+
+def apply_constraints(v, cholesky_yy, y):
+    """LOBPCG's apply_constraints (lobpcg/algorithm.rs:60-73): v -= y (L_yy^-1 (y^T v))
+    in place; cholesky_yy is the lower Cholesky factor of y^T y."""
+    for t in (v, cholesky_yy, y):
+        _check(t)
+    n, k = v.shape
+    ky = y.shape[1]
+    if y.shape[0] != n or tuple(cholesky_yy.shape) != (ky, ky):
+        raise ValueError("shape mismatch")
+    _call("apply_constraints", v, n, k, ctypes.c_void_p(v.data_ptr()), v.stride(0), ky,
+          ctypes.c_void_p(y.data_ptr()), y.stride(0), ctypes.c_void_p(cholesky_yy.data_ptr()),
+          cholesky_yy.stride(0))
+    return v
+
+
 def set_profiling(enable):
     c = context()
     err = _abi.LlhError()

@@ -120,16 +120,18 @@ static void set_err(orc_err* e, int code, int64_t a, int64_t b, int64_t col) {
   }                                                                                    \
                                                                                        \
   /* solve_triangular_system_common, triangular.rs:95-128, with the dispatch of      \
-   * solve_triangular_system, triangular.rs:133-144, and diag_fn = a[i,i]            \
-   * (triangular.rs:166).  For each RHS column k, rows in order (Lower: 0..n,         \
-   * Upper: n-1..0): coeff = b[i,k] / a[i,i]; b[i,k] = coeff; then ndarray's          \
-   * scaled_add(-coeff, x) on the rest of the column, which is y = y + (alpha * x)   \
-   * element by element (ndarray 0.16 ArrayBase::scaled_add).  Lower uses             \
-   * a[i+1.., i] and b[i+1.., k]; Upper uses a[..i, i] and b[..i, k]. */              \
-  int orc_solve_triangular_inplace_##sfx(int64_t rows, int64_t cols, const T* a,       \
-                                         int64_t rs, int64_t cs, int64_t brows,        \
-                                         int64_t bcols, T* b, int64_t brs,             \
-                                         int64_t bcs, int uplo, orc_err* e) {          \
+   * solve_triangular_system, triangular.rs:133-144.  For each RHS column k, rows    \
+   * in order (Lower: 0..n, Upper: n-1..0): coeff = b[i,k] / diag_fn(i);             \
+   * b[i,k] = coeff; then ndarray's scaled_add(-coeff, x) on the rest of the column, \
+   * which is y = y + (alpha * x) element by element (ndarray 0.16                   \
+   * ArrayBase::scaled_add).  Lower uses a[i+1.., i] and b[i+1.., k]; Upper uses     \
+   * a[..i, i] and b[..i, k], so a's diagonal is read only through diag_fn.          \
+   * diag_fn is a[i,i] (triangular.rs:166) when dg is NULL, else dg[i * ds], or its  \
+   * absolute value when absd (QR's |diag_i|, qr.rs:145-150, 172-177). */            \
+  static int solve_tri_common_##sfx(int64_t rows, int64_t cols, const T* a,          \
+                                    int64_t rs, int64_t cs, const T* dg, int64_t ds,  \
+                                    int absd, int64_t brows, int64_t bcols, T* b,     \
+                                    int64_t brs, int64_t bcs, int uplo, orc_err* e) { \
     set_err(e, ORC_OK, 0, 0, -1);                                                      \
     if (check_square_##sfx(rows, cols, e)) return ORC_NOT_SQUARE;                      \
     const int64_t n = rows;                                                            \
@@ -140,7 +142,8 @@ static void set_err(orc_err* e, int code, int64_t a, int64_t b, int64_t col) {
     for (int64_t k = 0; k < bcols; ++k) {                                              \
       for (int64_t t = 0; t < n; ++t) {                                                \
         const int64_t i = (uplo == ORC_UPPER) ? (n - 1 - t) : t;                       \
-        const T diag = AT(a, i, i, rs, cs);                                            \
+        T diag = dg ? dg[i * ds] : AT(a, i, i, rs, cs);                                \
+        if (dg && absd && !MUT(9) && diag < (T)0) diag = -diag;                        \
         const T coeff = AT(b, i, k, brs, bcs) / diag;                                  \
         AT(b, i, k, brs, bcs) = coeff;                                                 \
         const T alpha = MUT(5) ? coeff : -coeff;                                       \
@@ -154,6 +157,26 @@ static void set_err(orc_err* e, int code, int64_t a, int64_t b, int64_t col) {
       }                                                                                \
     }                                                                                  \
     return ORC_OK;                                                                     \
+  }                                                                                    \
+                                                                                       \
+  /* SolveTriangularInplace::solve_triangular_inplace, triangular.rs:147-169. */      \
+  int orc_solve_triangular_inplace_##sfx(int64_t rows, int64_t cols, const T* a,       \
+                                         int64_t rs, int64_t cs, int64_t brows,        \
+                                         int64_t bcols, T* b, int64_t brs,             \
+                                         int64_t bcs, int uplo, orc_err* e) {          \
+    return solve_tri_common_##sfx(rows, cols, a, rs, cs, NULL, 0, 0, brows, bcols, b,  \
+                                  brs, bcs, uplo, e);                                  \
+  }                                                                                    \
+                                                                                       \
+  /* solve_triangular_system with a diag_fn that reads a separate vector             \
+   * (triangular.rs:133-144; the QR callers, qr.rs:145-150 and 172-177). */           \
+  int orc_solve_triangular_system_##sfx(int64_t rows, int64_t cols, const T* a,        \
+                                        int64_t rs, int64_t cs, const T* dg,           \
+                                        int64_t ds, int absd, int64_t brows,           \
+                                        int64_t bcols, T* b, int64_t brs, int64_t bcs, \
+                                        int uplo, orc_err* e) {                        \
+    return solve_tri_common_##sfx(rows, cols, a, rs, cs, dg, ds, absd, brows, bcols,   \
+                                  b, brs, bcs, uplo, e);                               \
   }                                                                                    \
                                                                                        \
   /* SolveCInplace::solvec_inplace, cholesky.rs:136-144: A := dirty factor, then     \

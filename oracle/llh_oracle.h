@@ -51,6 +51,11 @@ typedef struct {
                                          int64_t rs, int64_t cs, int64_t brows,     \
                                          int64_t bcols, T* b, int64_t brs,          \
                                          int64_t bcs, int uplo, orc_err* e);        \
+  int orc_solve_triangular_system_##sfx(int64_t rows, int64_t cols, const T* a,     \
+                                        int64_t rs, int64_t cs, const T* dg,        \
+                                        int64_t ds, int absd, int64_t brows,        \
+                                        int64_t bcols, T* b, int64_t brs,           \
+                                        int64_t bcs, int uplo, orc_err* e);         \
   int orc_solvec_inplace_##sfx(int64_t rows, int64_t cols, T* a, int64_t rs,        \
                                int64_t cs, int64_t brows, int64_t bcols, T* b,      \
                                int64_t brs, int64_t bcs, orc_err* e);               \

@@ -18,7 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 OK, NOT_SQUARE, NOT_POSITIVE_DEFINITE, WRONG_ROWS = 0, 1, 2, 3
 UPPER, LOWER = 0, 1
-MUTANTS = tuple(range(1, 9))
+MUTANTS = tuple(range(1, 10))
 
 
 class OrcErr(ctypes.Structure):
@@ -73,6 +73,8 @@ class Oracle:
             f("is_triangular").argtypes = [i64, i64, ptr, i64, i64, cint]
             f("solve_triangular_inplace").argtypes = [
                 i64, i64, ptr, i64, i64, i64, i64, ptr, i64, i64, cint, eptr]
+            f("solve_triangular_system").argtypes = [
+                i64, i64, ptr, i64, i64, ptr, i64, cint, i64, i64, ptr, i64, i64, cint, eptr]
             f("solvec_inplace").argtypes = [i64, i64, ptr, i64, i64, i64, i64, ptr, i64, i64, eptr]
             f("invc_inplace").argtypes = [i64, i64, ptr, i64, i64, ptr, eptr]
 
@@ -102,6 +104,15 @@ class Oracle:
         e = OrcErr()
         _check(self._fn("solve_triangular_inplace", a)(*_view(a), *_view(b), uplo,
                                                         ctypes.byref(e)), e)
+        return b
+
+    def solve_triangular_system(self, a, b, uplo, diag, abs_diag=True):
+        """solve_triangular_system with diag_fn(i) = |diag[i]| (or diag[i]), in place in b."""
+        assert a.dtype == b.dtype == diag.dtype and diag.ndim == 1
+        e = OrcErr()
+        ds = diag.strides[0] // diag.itemsize
+        _check(self._fn("solve_triangular_system", a)(*_view(a), diag.ctypes.data, ds, int(bool(abs_diag)),
+                                                       *_view(b), uplo, ctypes.byref(e)), e)
         return b
 
     def solvec_inplace(self, a, b):

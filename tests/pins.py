@@ -167,6 +167,41 @@ def pin_failures(impl, status_codes=(1, 2, 3)):
     except Exception:  # noqa: BLE001
         bad.append("known_failure_raised")
 
+    # solve_triangular_system with diag_fn = |diag[i]| (QR, qr.rs:145-150, 172-177)
+    if hasattr(impl, "solve_triangular_system"):
+        rng_q = np.random.default_rng(11)
+        try:
+            a = arr("known_failure_a")
+            xk = arr("known_failure_x")
+            own = np.ascontiguousarray(np.diag(a))
+            check("system_own_diag_is_solve",
+                  np.array_equal(impl.solve_triangular_system(a, seq_matmul(a, xk), UPPER, own, False),
+                                 impl.solve_triangular(a, seq_matmul(a, xk), UPPER)))
+            for n in (1, 3, 6, 9):
+                r = np.triu(rng_q.integers(-3, 4, size=(n, n)).astype(np.float64), 1)
+                d = rng_q.choice([-1.0, 1.0], size=n)
+                xs = rng_q.integers(-5, 6, size=(n, 3)).astype(np.float64)
+                eff = r + np.eye(n)  # pivots |d_i| = 1
+                poisoned = r + np.diag(np.full(n, np.nan))  # a's diagonal must not be read
+                b = eff @ xs
+                check(f"system_upper_abs_{n}",
+                      np.array_equal(impl.solve_triangular_system(poisoned, np.array(b), UPPER, d, True), xs))
+                # QR's solve_tr_into: R^T (a transposed view) as Lower
+                bt = eff.T @ xs
+                check(f"system_lower_transposed_{n}",
+                      np.array_equal(impl.solve_triangular_system(poisoned.T, np.array(bt), LOWER, d, True), xs))
+                signed = r + np.diag(d)
+                check(f"system_signed_{n}",
+                      np.array_equal(impl.solve_triangular_system(poisoned, signed @ xs, UPPER, d, False), xs))
+        except Exception:  # noqa: BLE001
+            bad.append("system_raised")
+        code, e = _err_code(impl.solve_triangular_system, arr("solve_error_square"), arr("solve_error_b"),
+                            UPPER, np.ones(2), True)
+        check("system_wrong_rows", code == WRONG_ROWS and (e.a, e.b) == (2, 1))
+        code, _ = _err_code(impl.solve_triangular_system, arr("solve_error_nonsquare"),
+                            arr("solve_error_nonsquare"), LOWER, np.ones(3), True)
+        check("system_not_square", code == NOT_SQUARE)
+
     # Exact-integer constructions (closed form: the factor is L0, the solution X).
     rng = np.random.default_rng(7)
     for n in (2, 3, 5, 8, 10):

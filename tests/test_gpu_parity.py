@@ -301,3 +301,21 @@ def test_dev_solvec_and_potrf(n):
     dev.potrf(tc, zero_upper=True)
     c = tc.cpu().numpy()
     assert np.all(np.triu(c, 1) == 0) and np.array_equal(c, np.tril(fac))
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-4)])
+@pytest.mark.parametrize("n", [5, 70, 300])
+def test_solve_triangular_system_matches_oracle(dtype, tol, n):
+    """QR's back-substitutions: R x = b with pivots |diag_i| from a separate
+    vector (Upper), and R^T x = b through a transposed view (Lower); a's own
+    diagonal is poisoned with NaN and must not be read."""
+    rng = np.random.default_rng(n)
+    r = (np.triu(rng.uniform(-1, 1, (n, n)), 1) / np.sqrt(n)).astype(dtype)
+    d = (rng.uniform(0.5, 2.0, n) * rng.choice([-1, 1], n)).astype(dtype)
+    r[np.diag_indices(n)] = np.nan
+    b = rng.standard_normal((n, 4)).astype(dtype)
+    for a, uplo, oracle_uplo in ((r, llh.UPLO.Upper, 0), (r.T, llh.UPLO.Lower, 1)):
+        x = llh.solve_triangular_system(a, np.array(b), uplo, d, True)
+        x_ref = orc.solve_triangular_system(a, np.array(b), oracle_uplo, d, True)
+        assert np.all(np.isfinite(x))
+        assert np.abs(x - x_ref).max() <= tol * max(1.0, np.abs(x_ref).max()), (n, uplo)
