@@ -161,6 +161,90 @@ def batched_leg(dev, torch, args):
                                  "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}}
 
 
+def next_rows_leg(dev, torch, args):
+    """LOBPCG's Cholesky consumers and QR's back-substitution (the rows beside the
+    solve path): device orthonormalize and apply_constraints on a tall block
+    (n = 2^20, k = 64, f64), and the pivot-vector triangular solve through the
+    plain host form (n = 2048, 16 RHS).  CPU baselines: the oracle's cholesky and
+    solve_triangular (C, 1 thread) with the Gram products in numpy on 1 thread,
+    on a 65536-row sample scaled to 2^20 rows."""
+    import numpy as np
+    import threadpoolctl
+    import linfa_linalg_hip as llh
+    from oracle import oracle as orc
+    out = {}
+    n, k, ky = 1 << 20, 64, 64
+    g = torch.Generator(device="cuda").manual_seed(args.seed + 11)
+    v0 = torch.randn(n, k, dtype=torch.float64, device="cuda", generator=g)
+    y = torch.randn(n, ky, dtype=torch.float64, device="cuda", generator=g)
+    l_yy = (y.T @ y).contiguous()
+    dev.potrf(l_yy, zero_upper=True)
+    for name, fn, flops, nbytes in (
+            ("orthonormalize", lambda v: dev.orthonormalize(v), 4.0 * n * k * k + k ** 3 / 3.0, 24.0 * n * k),
+            ("apply_constraints", lambda v: dev.apply_constraints(v, l_yy, y), 4.0 * n * k * ky,
+             8.0 * n * (2 * ky + 3 * k))):
+        v = v0.clone()
+        fn(v)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(3, args.steps)):
+            v.copy_(v0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn(v)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        out[name] = {"workload": f"n={n} k={k}" + (f" ky={ky}" if name == "apply_constraints" else "") + " f64",
+                     "ms": round(t * 1e3, 3), "tflops": round(flops / t / 1e12, 3),
+                     "gbs_algorithmic": round(nbytes / t / 1e9, 1),
+                     "roofline_frac": round(max(flops / (PEAK_TFLOPS["f64"] * 1e12), nbytes / (PEAK_HBM_GBS * 1e9))
+                                            / t, 4)}
+        log(f"bench: {name} {out[name]}")
+    # CPU baseline on a sample of the rows
+    ns = 65536
+    rng = np.random.default_rng(args.seed + 12)
+    vs = rng.standard_normal((ns, k))
+    with threadpoolctl.threadpool_limits(1):
+        t0 = time.perf_counter()
+        gm = vs.T @ vs
+        lm = orc.cholesky(gm)
+        orc.solve_triangular(lm, np.array(vs.T), 1)
+        t_orth = time.perf_counter() - t0
+        ys = rng.standard_normal((ns, ky))
+        lys = orc.cholesky(ys.T @ ys)
+        t0 = time.perf_counter()
+        gy = ys.T @ vs
+        u = orc.solve_triangular(lys, gy, 1)
+        vs2 = vs - ys @ u
+        t_ac = time.perf_counter() - t0
+    scale = n / ns
+    out["orthonormalize"]["cpu_s_scaled"] = round(t_orth * scale, 3)
+    out["apply_constraints"]["cpu_s_scaled"] = round(t_ac * scale, 3)
+    out["cpu_sample"] = (f"oracle cholesky + solve_triangular (C, 1 thread), numpy Gram products on 1 thread, "
+                         f"{ns} rows, scaled x{scale:.0f}")
+    del v0, y, l_yy, vs2
+    # QR back-substitution: R x = b with pivots |diag_i|, plain host form
+    nq, mq = 2048, 16
+    r = np.triu(rng.uniform(-1, 1, (nq, nq)), 1) / np.sqrt(nq)
+    d = rng.uniform(0.5, 2.0, nq) * rng.choice([-1.0, 1.0], nq)
+    bq = rng.standard_normal((nq, mq))
+    llh.solve_triangular_system(r, np.array(bq), llh.UPLO.Upper, d, True)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        llh.solve_triangular_system(r, np.array(bq), llh.UPLO.Upper, d, True)
+        ts.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    orc.solve_triangular_system(r, np.array(bq), 0, d, True)
+    t_cpu = time.perf_counter() - t0
+    out["qr_solve"] = {"workload": f"R x = b, n={nq}, {mq} RHS, pivots |diag_i| (llh_solve_triangular_system_d, "
+                                   "host arrays through the boundary)",
+                       "ms": round(float(np.median(ts)) * 1e3, 3), "cpu_oracle_ms": round(t_cpu * 1e3, 3)}
+    log(f"bench: qr_solve {out['qr_solve']}")
+    return out
+
+
 def serving_leg(dev, torch, args):
     """Poisson arrivals of solve requests at several offered rates: n log-uniform
     in [16, 4096], A = B B^T / n + I with B uniform in [-1, 1] (pre-staged in HBM,
@@ -354,6 +438,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-next", action="store_true")
     ap.add_argument("--rocprof-summary", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="summarise rocprofv3 runs of this benchmark into profiles/ and exit")
     ap.add_argument("--rocprof-command", default="")
@@ -552,6 +637,10 @@ def main():
         log(f"bench: batched {batched['solves_per_s']} solves/s ({batched['ms']} ms), "
             f"plain {batched['plain_per_matrix']}")
 
+    next_rows = None
+    if not args.no_next:
+        next_rows = next_rows_leg(dev, torch, args)
+
     serving = None
     if not args.no_serve:
         serving = serving_leg(dev, torch, args)
@@ -585,6 +674,7 @@ def main():
         "c1": c1,
         "batched": batched,
         "serving": serving,
+        "lobpcg_qr": next_rows,
         "kernels": kernels,
         "check": check,
     }
