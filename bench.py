@@ -147,18 +147,57 @@ def batched_leg(dev, torch, args):
     plain_rate = len(work) / (t1 - t0)
     frac = float(np.sum(sizes[sample].astype(np.float64) ** 3)) / float(np.sum(n64 ** 3))
     plain_rate_scaled = len(work) / ((t1 - t0) / frac * len(work) / cnt) if frac > 0 else None
+    cfg64 = plan.config
     plan.close()
+    del work, mats
+    # the same systems with FP32 factors and f64 refinement (f64 buffers, A kept), one graph launch
+    plan32 = dev.BatchPlan(sizes, rhs, prec="f32ir", ir_iterations=2)
+    assert np.array_equal(plan32.a_off, plan.a_off) and np.array_equal(plan32.b_off, plan.b_off)
+    a.copy_(a0)
+    b.copy_(b0)
+    log("bench: batched FP32+IR autotune")
+    tuned32 = plan32.autotune(a, b, status)
+    times32 = []
+    for rep in range(1 + args.steps):
+        b.copy_(b0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan32.solve(a, b, status)
+        t1 = time.perf_counter()
+        if rep:
+            times32.append(t1 - t0)
+    t32 = float(np.median(times32))
+    ok32 = int((status == -1).sum().item())
+    xs32 = b.cpu().numpy()
+    worst32 = 0.0
+    for i in rng.choice(cnt, 40, replace=False):
+        s, m = int(sizes[i]), int(rhs[i])
+        o, ld = int(plan32.a_off[i]), int(plan32.lda[i])
+        am = a0[o:o + s * ld].view(s, ld)[:, :s].cpu().numpy()
+        bm = b0[int(plan32.b_off[i]):int(plan32.b_off[i]) + s * m].view(s, m).cpu().numpy()
+        xr = orc.solvec(np.array(am), bm)
+        x = xs32[int(plan32.b_off[i]):int(plan32.b_off[i]) + s * m].reshape(s, m)
+        worst32 = max(worst32, float(np.linalg.norm(x - xr) / np.linalg.norm(xr)))
+    fp32 = {"workload": f"the same {cnt} systems: f32 factors and solves, 2 f64 refinement steps, f64 buffers, "
+                        "one hipGraph launch",
+            "solves_per_s": round(cnt / t32, 1), "ms": round(t32 * 1e3, 3),
+            "gflops": round(flops / t32 / 1e9, 1), "speedup_vs_f64": round(t / t32, 3),
+            "autotuned_ms": round(tuned32, 3), "config": plan32.config, "ok": ok32,
+            "max_rel_err_vs_oracle_sample": worst32}
+    log(f"bench: batched FP32+IR {fp32}")
+    plan32.close()
     return {"workload": f"{cnt} SPD systems, n log-uniform [8,512], 1-8 RHS, f64, one hipGraph launch",
             "solves_per_s": round(cnt / t, 1), "ms": round(t * 1e3, 3), "gflops": round(flops / t / 1e9, 1),
             "gbs_algorithmic": round(byts / t / 1e9, 1), "roofline_frac": round(roof_t / t, 4),
             "roofline_bound": "mfma" if flops / (PEAK_TFLOPS["f64"] * 1e12) > byts / (PEAK_HBM_GBS * 1e9)
-            else "hbm", "config": plan.config if False else None, "autotuned_ms": round(tuned_ms, 3),
+            else "hbm", "config": cfg64, "autotuned_ms": round(tuned_ms, 3),
             "ok": ok, "max_rel_err_vs_oracle_sample": worst,
             "plain_per_matrix": {"solves_per_s_sample": round(plain_rate, 1),
                                  "solves_per_s_same_mix_est": round(plain_rate_scaled, 1) if plain_rate_scaled
                                  else None,
                                  "p50_ms": round(float(np.median(lat)) * 1e3, 3),
-                                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}}
+                                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+            "fp32_ir": fp32}
 
 
 def next_rows_leg(dev, torch, args):

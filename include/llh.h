@@ -204,8 +204,9 @@ LLH_DEV_DECL(float, s)
  * while the factor is used.  A factor is bound to its context's stream.
  * ------------------------------------------------------------------------- */
 typedef enum llh_precision {
-  LLH_PREC_F64 = 0, /* plain f64 factor, no refinement */
-  LLH_PREC_F32 = 1  /* f32 trailing updates on MFMA, f64 diagonal blocks, f64 refinement */
+  LLH_PREC_F64 = 0,         /* plain f64 factor, no refinement */
+  LLH_PREC_F32 = 1,         /* f32 trailing updates on MFMA, f64 diagonal blocks, f64 refinement */
+  LLH_PREC_F32_REFINED = 2  /* batch plans only: f64 buffers, f32 factor and solves, f64 refinement */
 } llh_precision;
 
 enum { LLH_FACTOR_FALLBACK_F64 = 1 }; /* on f32 breakdown (pivot <= 0) or stalled refinement, redo in f64 */
@@ -268,11 +269,17 @@ llh_status llh_dev_solve_mixed(llh_context* ctx, llh_precision prec, size_t n, c
 typedef struct llh_batch_plan llh_batch_plan;
 
 typedef struct llh_batch_config {
-  int32_t small_limit; /* systems with n <= small_limit take the LDS path (<= 128 f64, <= 176 f32) */
-  int32_t small_waves; /* resident workgroups per CU of the persistent small-system kernels */
+  int32_t small_limit;   /* systems with n <= small_limit take the LDS path (<= 128 f64, <= 176 f32) */
+  int32_t small_waves;   /* resident workgroups per CU of the persistent small-system kernels */
+  int32_t ir_iterations; /* LLH_PREC_F32_REFINED: refinement steps (0 = 2) */
 } llh_batch_config;
 
-/* prec: LLH_PREC_F64 (buffers of double) or LLH_PREC_F32 (buffers of float). */
+/* prec: LLH_PREC_F64 (buffers of double), LLH_PREC_F32 (buffers of float), or
+ * LLH_PREC_F32_REFINED (buffers of double; every A_i is factored in f32 from a
+ * copy, B_i is solved in f32 and then refined ir_iterations times with an f64
+ * residual from the lower triangle of the f64 A_i, which is left unchanged; the
+ * whole sequence is still one graph launch).  A system whose f32 factor fails
+ * reports that row in status and is not refined. */
 llh_status llh_batch_plan_create(llh_context* ctx, llh_precision prec, size_t count, const int32_t* n,
                                  const int32_t* m, const llh_batch_config* config, llh_batch_plan** out,
                                  llh_error* err);
@@ -282,7 +289,8 @@ size_t llh_batch_b_elems(const llh_batch_plan* plan);
 llh_status llh_batch_layout(const llh_batch_plan* plan, uint64_t* a_off, int32_t* lda, uint64_t* b_off,
                             llh_error* err);
 llh_batch_config llh_batch_get_config(const llh_batch_plan* plan);
-/* One graph launch: factors every A_i and solves every B_i in place. */
+/* One graph launch: factors every A_i and solves every B_i in place (with
+ * LLH_PREC_F32_REFINED, A_i is left as it was). */
 llh_status llh_batch_solve(llh_context* ctx, llh_batch_plan* plan, void* a, void* b, int32_t* status,
                            llh_error* err);
 /* Times the candidate configurations on copies of (a, b) (restored after each

@@ -59,6 +59,8 @@ struct llh_batch_plan {
   std::vector<SmallLaunch> small;
   std::vector<Step> steps;
   std::vector<std::pair<int, int>> bwd;  // StepWork ranges of the back-solve steps
+  int res_off = 0, nres = 0;             // StepWork range of the residual blocks (refined mode)
+  int nclasses = 0;                      // small-system launches per pass
   int ncounters = 0;
   size_t linv_elems = 0;
 
@@ -67,7 +69,14 @@ struct llh_batch_plan {
   void* d_step = nullptr;
   void* d_tile = nullptr;
   void* d_base = nullptr;
+  void* d_mbase = nullptr;  // MixedBase (refined mode)
   void* d_linv = nullptr;
+  // refined mode: f32 copy of A, f32 right-hand side / correction, f64 copies
+  // of B and of the residual
+  float* a32 = nullptr;
+  float* x32 = nullptr;
+  double* bk64 = nullptr;
+  double* r64 = nullptr;
   int32_t* d_counters = nullptr;
   size_t linv_bytes = 0;
 
@@ -76,6 +85,8 @@ struct llh_batch_plan {
   hipEvent_t fork = nullptr, join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  BatchBase<float> h_base32{};
+  MixedBase h_mbase{};
 };
 
 namespace {
@@ -89,7 +100,11 @@ namespace {
 bool bad_ctx(llh_context* ctx) { return ctx == nullptr || ctx->magic != llh_context::kMagic; }
 bool bad_plan(const llh_batch_plan* p) { return p == nullptr || p->magic != llh_batch_plan::kMagic; }
 
+// Element size of the caller's buffers, and of the arithmetic of the factor.
 size_t esize(const llh_batch_plan* p) { return p->prec == LLH_PREC_F32 ? 4 : 8; }
+size_t compute_esize(const llh_batch_plan* p) { return p->prec == LLH_PREC_F64 ? 8 : 4; }
+bool refined(const llh_batch_plan* p) { return p->prec == LLH_PREC_F32_REFINED; }
+int ir_steps(const llh_batch_plan* p) { return p->cfg.ir_iterations > 0 ? p->cfg.ir_iterations : 2; }
 
 void free_schedule(llh_batch_plan* p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
@@ -104,35 +119,66 @@ void free_schedule(llh_batch_plan* p) {
   p->d_counters = nullptr;
 }
 
+// One pass over the batch: pass 0 factors and solves; a later pass (the
+// refinement of the mixed mode) solves again against the stored factors.  The
+// small systems run on the side stream, the blocked steps on the main stream;
+// each pass has its own work-queue counters.
 template <typename T>
-hipError_t record(llh_batch_plan* p, hipStream_t s) {
+hipError_t record_pass(llh_batch_plan* p, hipStream_t s, int pass) {
   const auto* base = static_cast<const BatchBase<T>*>(p->d_base);
   const auto* items = static_cast<const BatchItem*>(p->d_items);
   const auto* sw = static_cast<const StepWork*>(p->d_step);
   const auto* tw = static_cast<const TileWork*>(p->d_tile);
+  hipError_t e;
+  if ((e = hipEventRecord(p->fork, s)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(p->side, p->fork, 0)) != hipSuccess) return e;
+  for (const auto& sl : p->small)
+    if ((e = launch_small_batch<T>(base, items + sl.first, sl.count, sl.nmax, sl.grid,
+                                   sl.counter + pass * p->nclasses, p->side, pass == 0 ? 1 : 2)) != hipSuccess)
+      return e;
+  for (const auto& st : p->steps) {
+    if (pass == 0) {
+      if ((e = launch_grp_diag_fast<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
+      if ((e = launch_grp_panel<T>(base, items, tw + st.panel_off, st.npanel, s)) != hipSuccess) return e;
+    }
+    if ((e = launch_grp_fwd<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
+    if (pass == 0)
+      if ((e = launch_grp_syrk<T>(base, items, tw + st.syrk_off, st.nsyrk, s)) != hipSuccess) return e;
+  }
+  for (const auto& bw : p->bwd)
+    if ((e = launch_grp_bwd<T>(base, items, sw + bw.first, bw.second, s)) != hipSuccess) return e;
+  if ((e = hipEventRecord(p->join, p->side)) != hipSuccess) return e;
+  return hipStreamWaitEvent(s, p->join, 0);
+}
+
+template <typename T>
+hipError_t record(llh_batch_plan* p, hipStream_t s) {
+  const auto* base = static_cast<const BatchBase<T>*>(p->d_base);
   const int64_t cnt = (int64_t)p->count;
   const int64_t initn = std::max<int64_t>(cnt, p->ncounters);
   hipLaunchKernelGGL((batch_init_from_base<T>), dim3((initn + 255) / 256), dim3(256), 0, s, base, cnt,
                      p->ncounters);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // Small systems on the side stream, the blocked steps on the main stream.
-  if ((e = hipEventRecord(p->fork, s)) != hipSuccess) return e;
-  if ((e = hipStreamWaitEvent(p->side, p->fork, 0)) != hipSuccess) return e;
-  for (const auto& sl : p->small)
-    if ((e = launch_small_batch<T>(base, items + sl.first, sl.count, sl.nmax, sl.grid, sl.counter, p->side)) !=
-        hipSuccess)
-      return e;
-  for (const auto& st : p->steps) {
-    if ((e = launch_grp_diag_fast<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
-    if ((e = launch_grp_panel<T>(base, items, tw + st.panel_off, st.npanel, s)) != hipSuccess) return e;
-    if ((e = launch_grp_fwd<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
-    if ((e = launch_grp_syrk<T>(base, items, tw + st.syrk_off, st.nsyrk, s)) != hipSuccess) return e;
+  if (!refined(p)) {
+    if ((e = record_pass<T>(p, s, 0)) != hipSuccess) return e;
+  } else {
+    // f32 factor and solve from copies, then refinement in f64:
+    // x = x32; repeat { r = b - A x (f64); x32 = solve32(r); x += x32 }
+    const auto* mb = static_cast<const MixedBase*>(p->d_mbase);
+    const auto* items = static_cast<const BatchItem*>(p->d_items);
+    const auto* sw = static_cast<const StepWork*>(p->d_step);
+    if ((e = launch_mixed_flat(mb, 0, (int64_t)p->a_elems, s)) != hipSuccess) return e;
+    if ((e = launch_mixed_flat(mb, 1, (int64_t)p->b_elems, s)) != hipSuccess) return e;
+    if ((e = record_pass<T>(p, s, 0)) != hipSuccess) return e;
+    if ((e = launch_mixed_flat(mb, 2, (int64_t)p->b_elems, s)) != hipSuccess) return e;
+    for (int it = 1; it <= ir_steps(p); ++it) {
+      if ((e = launch_grp_residual(mb, items, sw + p->res_off, p->nres, s)) != hipSuccess) return e;
+      if ((e = launch_mixed_flat(mb, 3, (int64_t)p->b_elems, s)) != hipSuccess) return e;
+      if ((e = record_pass<T>(p, s, it)) != hipSuccess) return e;
+      if ((e = launch_mixed_flat(mb, 4, (int64_t)p->b_elems, s)) != hipSuccess) return e;
+    }
   }
-  for (const auto& bw : p->bwd)
-    if ((e = launch_grp_bwd<T>(base, items, sw + bw.first, bw.second, s)) != hipSuccess) return e;
-  if ((e = hipEventRecord(p->join, p->side)) != hipSuccess) return e;
-  if ((e = hipStreamWaitEvent(s, p->join, 0)) != hipSuccess) return e;
   hipLaunchKernelGGL((batch_finish_from_base<T>), dim3((cnt + 255) / 256), dim3(256), 0, s, base, cnt);
   return hipGetLastError();
 }
@@ -141,7 +187,7 @@ hipError_t record(llh_batch_plan* p, hipStream_t s) {
 // lists and captures the graph on the context's stream.
 llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
   free_schedule(p);
-  const int smax = p->prec == LLH_PREC_F32 ? kSmallMaxS : kSmallMaxD;
+  const int smax = p->prec == LLH_PREC_F64 ? kSmallMaxD : kSmallMaxS;
   p->cfg.small_limit = std::max(0, std::min(p->cfg.small_limit, smax));
   p->cfg.small_waves = std::max(1, std::min(p->cfg.small_waves, 16));
   std::vector<int> order(p->count);
@@ -176,7 +222,8 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
     pos += cls.size();
   }
   (void)pos;
-  p->ncounters = std::max(counter, 1);
+  p->nclasses = counter;
+  p->ncounters = std::max(counter * (refined(p) ? 1 + ir_steps(p) : 1), 1);
   // larger systems: grouped blocked steps
   const int med_first = (int)items.size();
   int64_t linv = 0;
@@ -221,6 +268,12 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
       if (items[q].n > k0) sw.push_back({q, k0});
     p->bwd.push_back({off, (int)sw.size() - off});
   }
+  // refined mode: the residual of every system, 64 rows per workgroup
+  p->res_off = (int)sw.size();
+  if (refined(p))
+    for (int q = 0; q < (int)items.size(); ++q)
+      for (int r0 = 0; r0 < items[q].n; r0 += 64) sw.push_back({q, r0});
+  p->nres = (int)sw.size() - p->res_off;
   auto upload = [&](void** dst, const void* src, size_t bytes) -> hipError_t {
     hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 64));
     if (e != hipSuccess) return e;
@@ -230,8 +283,8 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
   CHECK_HIP(upload(&p->d_items, items.data(), items.size() * sizeof(BatchItem)), "batch items");
   CHECK_HIP(upload(&p->d_step, sw.data(), sw.size() * sizeof(StepWork)), "batch work");
   CHECK_HIP(upload(&p->d_tile, tw.data(), tw.size() * sizeof(TileWork)), "batch tiles");
-  CHECK_HIP(hipMalloc(&p->d_counters, 64 * sizeof(int32_t)), "batch counters");
-  const size_t lb = std::max<size_t>(p->linv_elems * esize(p), 64);
+  CHECK_HIP(hipMalloc(&p->d_counters, std::max(64, p->ncounters) * sizeof(int32_t)), "batch counters");
+  const size_t lb = std::max<size_t>(p->linv_elems * compute_esize(p), 64);
   if (p->linv_bytes < lb) {
     if (p->d_linv) (void)hipFree(p->d_linv);
     p->d_linv = nullptr;
@@ -244,7 +297,7 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
   hipStream_t s = p->cap;
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
   CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-  hipError_t e = p->prec == LLH_PREC_F32 ? record<float>(p, s) : record<double>(p, s);
+  hipError_t e = p->prec == LLH_PREC_F64 ? record<double>(p, s) : record<float>(p, s);
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(s, &g);
   if (e != hipSuccess) {
@@ -260,7 +313,16 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
 llh_status run(llh_context* ctx, llh_batch_plan* p, void* a, void* b, int32_t* status, bool sync,
                llh_error* err) {
   // base pointers (outside the graph), then one graph launch
-  if (p->prec == LLH_PREC_F32) {
+  if (refined(p)) {
+    // host staging of the two bases lives in the plan: the copies are async
+    p->h_base32 = BatchBase<float>{p->a32, p->x32, static_cast<float*>(p->d_linv), status, p->d_counters};
+    p->h_mbase = MixedBase{static_cast<const double*>(a), static_cast<double*>(b), p->a32, p->x32, p->bk64, p->r64,
+                           status};
+    CHECK_HIP(hipMemcpyAsync(p->d_base, &p->h_base32, sizeof(p->h_base32), hipMemcpyHostToDevice, ctx->stream),
+              "batch base");
+    CHECK_HIP(hipMemcpyAsync(p->d_mbase, &p->h_mbase, sizeof(p->h_mbase), hipMemcpyHostToDevice, ctx->stream),
+              "batch base");
+  } else if (p->prec == LLH_PREC_F32) {
     BatchBase<float> bb{static_cast<float*>(a), static_cast<float*>(b), static_cast<float*>(p->d_linv), status,
                         p->d_counters};
     CHECK_HIP(hipMemcpyAsync(p->d_base, &bb, sizeof(bb), hipMemcpyHostToDevice, ctx->stream), "batch base");
@@ -284,7 +346,8 @@ llh_status llh_batch_plan_create(llh_context* ctx, llh_precision prec, size_t co
   if (out == nullptr) return invalid(err, "null output pointer");
   *out = nullptr;
   if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
-  if (prec != LLH_PREC_F64 && prec != LLH_PREC_F32) return invalid(err, "unknown precision");
+  if (prec != LLH_PREC_F64 && prec != LLH_PREC_F32 && prec != LLH_PREC_F32_REFINED)
+    return invalid(err, "unknown precision");
   if (count > 0 && (n == nullptr || m == nullptr)) return invalid(err, "null size arrays");
   if (count > (1u << 28)) return invalid(err, "too many systems");
   auto* p = new llh_batch_plan();
@@ -301,8 +364,7 @@ llh_status llh_batch_plan_create(llh_context* ctx, llh_precision prec, size_t co
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) p->cus = prop.multiProcessorCount;
-  p->cfg = config ? *config : llh_batch_config{prec == LLH_PREC_F32 ? kSmallMaxS : kSmallMaxD, 4};
-  if (config == nullptr) p->cfg.small_limit = prec == LLH_PREC_F32 ? kSmallMaxS : kSmallMaxD;
+  p->cfg = config ? *config : llh_batch_config{prec == LLH_PREC_F64 ? kSmallMaxD : kSmallMaxS, 4, 0};
   // packed layout: 256-byte aligned starts, lda rounded up to 4
   const size_t es = esize(p), align = 256 / es;
   size_t ao = 0, bo = 0;
@@ -323,6 +385,13 @@ llh_status llh_batch_plan_create(llh_context* ctx, llh_precision prec, size_t co
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&p->d_base, sizeof(BatchBase<double>));
+  if (e == hipSuccess && refined(p)) {
+    e = hipMalloc(&p->d_mbase, sizeof(MixedBase));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->a32), std::max<size_t>(ao, 16) * 4);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->x32), std::max<size_t>(bo, 16) * 4);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->bk64), std::max<size_t>(bo, 16) * 8);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->r64), std::max<size_t>(bo, 16) * 8);
+  }
   if (e != hipSuccess) {
     llh_batch_plan_destroy(p);
     return hip_fail(err, e, "batch plan");
@@ -349,7 +418,7 @@ llh_status llh_batch_layout(const llh_batch_plan* p, uint64_t* a_off, int32_t* l
 }
 
 llh_batch_config llh_batch_get_config(const llh_batch_plan* p) {
-  return bad_plan(p) ? llh_batch_config{0, 0} : p->cfg;
+  return bad_plan(p) ? llh_batch_config{0, 0, 0} : p->cfg;
 }
 
 llh_status llh_batch_solve(llh_context* ctx, llh_batch_plan* p, void* a, void* b, int32_t* status,
@@ -377,7 +446,7 @@ llh_status llh_batch_autotune(llh_context* ctx, llh_batch_plan* p, const void* a
   hipEvent_t t0 = nullptr, t1 = nullptr;
   (void)hipEventCreate(&t0);
   (void)hipEventCreate(&t1);
-  const int smax = p->prec == LLH_PREC_F32 ? kSmallMaxS : kSmallMaxD;
+  const int smax = p->prec == LLH_PREC_F64 ? kSmallMaxD : kSmallMaxS;
   const llh_batch_config keep = p->cfg;
   llh_batch_config best = keep;
   double best_t = 1e300;
@@ -387,7 +456,7 @@ llh_status llh_batch_autotune(llh_context* ctx, llh_batch_plan* p, const void* a
   for (int lim : limits) {
     for (int wv : waves) {
       if (lim > 64 && wv > 1) continue;  // one resident 128-class workgroup per CU fills its LDS
-      p->cfg = {lim, wv};
+      p->cfg = {lim, wv, keep.ir_iterations};
       st = build_schedule(ctx, p, err);
       if (st != LLH_OK) break;
       float ms_best = 1e30f;
@@ -436,6 +505,9 @@ void llh_batch_plan_destroy(llh_batch_plan* p) {
   free_schedule(p);
   if (p->d_linv) (void)hipFree(p->d_linv);
   if (p->d_base) (void)hipFree(p->d_base);
+  if (p->d_mbase) (void)hipFree(p->d_mbase);
+  for (void* q : {(void*)p->a32, (void*)p->x32, (void*)p->bk64, (void*)p->r64})
+    if (q) (void)hipFree(q);
   if (p->fork) (void)hipEventDestroy(p->fork);
   if (p->join) (void)hipEventDestroy(p->join);
   if (p->side) (void)hipStreamDestroy(p->side);

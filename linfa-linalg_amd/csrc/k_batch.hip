@@ -4,6 +4,8 @@
 // the planner built once, so a whole batch of uneven sizes runs as a fixed
 // sequence of launches (captured in one hipGraph).  A matrix whose status word
 // records a failed pivot is skipped by every later launch.
+#include <algorithm>
+
 #include "llh_common.h"
 #include "k_gemm_impl.h"
 #include "k_trsm_impl.h"
@@ -177,6 +179,58 @@ __global__ __launch_bounds__(256) void grp_bwd_kernel(const BatchBase<T>* __rest
   }
 }
 
+// Elementwise steps of the mixed-precision batch (pointers read from the device
+// base, so a replayed graph follows the caller's buffers).
+__global__ void mixed_flat_kernel(const MixedBase* __restrict__ base, int op, int64_t elems) {
+  const MixedBase mb = *base;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < elems;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    switch (op) {
+      case 0: mb.a32[i] = (float)mb.a64[i]; break;
+      case 1: {
+        const double v = mb.b64[i];
+        mb.x32[i] = (float)v;
+        mb.bk64[i] = v;
+        break;
+      }
+      case 2: mb.b64[i] = (double)mb.x32[i]; break;
+      case 3: mb.x32[i] = (float)mb.r64[i]; break;
+      default: mb.b64[i] += (double)mb.x32[i]; break;
+    }
+  }
+}
+
+// r = bk - A x for rows [r0, r0 + 64) of one system, in f64, from the lower
+// triangle of A (A[i][k] for k > i is read as A[k][i]).  Thread (row, q) takes
+// the columns q, q + 4, ... of its row, two at a time.
+__global__ __launch_bounds__(256) void grp_residual_kernel(const MixedBase* __restrict__ base,
+                                                           const BatchItem* __restrict__ items,
+                                                           const StepWork* __restrict__ work, int count) {
+  if ((int)blockIdx.x >= count) return;
+  const StepWork w = work[blockIdx.x];
+  const BatchItem it = items[w.item];
+  const MixedBase mb = *base;
+  if (mb.status[it.id] != kNoFail) return;
+  const int row = w.k0 + (threadIdx.x >> 2), q = threadIdx.x & 3;
+  if (row >= it.n) return;
+  const double* a = mb.a64 + it.a_off;
+  const double* x = mb.b64 + it.b_off;
+  const double* b = mb.bk64 + it.b_off;
+  double* r = mb.r64 + it.b_off;
+  const int m = it.m;
+  for (int c0 = q; c0 < m; c0 += 8) {
+    const int c1 = c0 + 4;
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = 0; k < it.n; ++k) {
+      const double av = k <= row ? a[(int64_t)row * it.lda + k] : a[(int64_t)k * it.lda + row];
+      s0 += av * x[(int64_t)k * m + c0];
+      if (c1 < m) s1 += av * x[(int64_t)k * m + c1];
+    }
+    r[(int64_t)row * m + c0] = b[(int64_t)row * m + c0] - s0;
+    if (c1 < m) r[(int64_t)row * m + c1] = b[(int64_t)row * m + c1] - s1;
+  }
+}
+
 __global__ void fill_status_kernel(int32_t* st, int64_t count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) st[i] = kNoFail;
@@ -201,6 +255,20 @@ LLH_GRP_LAUNCH(launch_grp_panel, grp_panel_kernel, TileWork)
 LLH_GRP_LAUNCH(launch_grp_syrk, grp_syrk_kernel, TileWork)
 LLH_GRP_LAUNCH(launch_grp_fwd, grp_fwd_kernel, StepWork)
 LLH_GRP_LAUNCH(launch_grp_bwd, grp_bwd_kernel, StepWork)
+
+hipError_t launch_mixed_flat(const MixedBase* mb, int op, int64_t elems, hipStream_t s) {
+  if (elems <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((elems + 255) / 256, 8192);
+  hipLaunchKernelGGL(mixed_flat_kernel, dim3((unsigned)blocks), dim3(256), 0, s, mb, op, elems);
+  return hipGetLastError();
+}
+
+hipError_t launch_grp_residual(const MixedBase* mb, const BatchItem* items, const StepWork* work, int count,
+                               hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(grp_residual_kernel, dim3(count), dim3(256), 0, s, mb, items, work, count);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill_status(int32_t* status, int64_t count, hipStream_t s) {
   if (count <= 0) return hipSuccess;

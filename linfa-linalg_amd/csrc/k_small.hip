@@ -55,7 +55,7 @@ __global__ __launch_bounds__(NT) void potrf_potrs_exact_batched(const SmallDesc<
 template <typename T, int NMAX, int NT, int MCH>
 __global__ __launch_bounds__(NT) void small_batch_kernel(const BatchBase<T>* __restrict__ base,
                                                          const BatchItem* __restrict__ items, int count,
-                                                         int counter) {
+                                                         int counter, int mode) {
   __shared__ SmallLds<T, NMAX, MCH> lds;
   __shared__ int s_next;
   const BatchBase<T> bb = *base;
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(NT) void small_batch_kernel(const BatchBase<T>* __r
     if (i >= count) return;
     const BatchItem it = items[i];
     SmallDesc<T> ds{bb.a + it.a_off, bb.b + it.b_off, it.lda, it.m, it.n, it.m, bb.status + it.id};
-    small_solve_one<T, NMAX, NT, MCH>(ds, 1, lds);
+    small_solve_one<T, NMAX, NT, MCH>(ds, mode, lds);
   }
 }
 
@@ -99,17 +99,18 @@ __global__ __launch_bounds__(256) void grp_diag_kernel(const BatchBase<T>* __res
 
 template <typename T>
 hipError_t launch_small_batch(const BatchBase<T>* base, const BatchItem* items, int count, int nmax, int grid,
-                              int counter, hipStream_t s) {
+                              int counter, hipStream_t s, int mode) {
   if (count <= 0) return hipSuccess;
   grid = std::max(1, std::min(grid, count));
   if (nmax <= 32) {
-    hipLaunchKernelGGL((small_batch_kernel<T, 32, 64, 8>), dim3(grid), dim3(64), 0, s, base, items, count, counter);
+    hipLaunchKernelGGL((small_batch_kernel<T, 32, 64, 8>), dim3(grid), dim3(64), 0, s, base, items, count, counter,
+                       mode);
   } else if (nmax <= 64) {
     hipLaunchKernelGGL((small_batch_kernel<T, 64, 128, 8>), dim3(grid), dim3(128), 0, s, base, items, count,
-                       counter);
+                       counter, mode);
   } else if (nmax <= small_max<T>()) {
     hipLaunchKernelGGL((small_batch_kernel<T, small_max_c<T>::value, 256, 8>), dim3(grid), dim3(256), 0, s, base,
-                       items, count, counter);
+                       items, count, counter, mode);
   } else {
     return hipErrorInvalidValue;
   }
@@ -125,9 +126,9 @@ hipError_t launch_grp_diag(const BatchBase<T>* base, const BatchItem* items, con
 }
 
 template hipError_t launch_small_batch<double>(const BatchBase<double>*, const BatchItem*, int, int, int, int,
-                                               hipStream_t);
+                                               hipStream_t, int);
 template hipError_t launch_small_batch<float>(const BatchBase<float>*, const BatchItem*, int, int, int, int,
-                                              hipStream_t);
+                                              hipStream_t, int);
 template hipError_t launch_grp_diag<double>(const BatchBase<double>*, const BatchItem*, const StepWork*, int,
                                             hipStream_t);
 template hipError_t launch_grp_diag<float>(const BatchBase<float>*, const BatchItem*, const StepWork*, int,

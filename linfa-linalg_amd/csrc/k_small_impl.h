@@ -118,6 +118,9 @@ struct SmallLds {
 
 // Factors ds.a (dirty, in place) and, when do_solve, solves ds.b in place in the
 // reference's operation order; ds.info receives kNoFail or the failing row.
+// do_solve == 2: ds.a already holds the factor (a solve against it, as the
+// refinement steps of a batched mixed-precision solve need); ds.info is read:
+// a matrix whose factor failed is skipped.
 template <typename T, int NMAX, int NT, int MCH>
 __device__ void small_solve_one(const SmallDesc<T>& ds, int do_solve, SmallLds<T, NMAX, MCH>& lds) {
   const int n = ds.n;
@@ -127,12 +130,21 @@ __device__ void small_solve_one(const SmallDesc<T>& ds, int do_solve, SmallLds<T
   T* Dg = lds.Dg;
   T* Bs = lds.Bs;
   T* Xs = lds.Xs;
-  const int fail = factor_exact<T, T, NT, NMAX>(ds.a, ds.lda, n, W, Ap, Dg);
-  __syncthreads();
-  write_back<T, T, NT>(ds.a, ds.lda, n, fail, W, Dg);
-  if (threadIdx.x == 0) ds.info[0] = fail < 0 ? kNoFail : fail;
-  if (fail >= 0 || !do_solve) return;
   const int tid = threadIdx.x;
+  if (do_solve == 2) {
+    if (ds.info[0] != kNoFail) return;
+    for (int e = tid; e < n * n; e += NT) {
+      const int j = e / n, k = e - j * n;
+      if (k < j) W[pk(n, j, k)] = ds.a[(int64_t)j * ds.lda + k];
+      else if (k == j) Dg[j] = ds.a[(int64_t)j * ds.lda + j];
+    }
+  } else {
+    const int fail = factor_exact<T, T, NT, NMAX>(ds.a, ds.lda, n, W, Ap, Dg);
+    __syncthreads();
+    write_back<T, T, NT>(ds.a, ds.lda, n, fail, W, Dg);
+    if (threadIdx.x == 0) ds.info[0] = fail < 0 ? kNoFail : fail;
+    if (fail >= 0 || !do_solve) return;
+  }
   // Solve in chunks of MCH right-hand-side columns; b is row-major n x m.
   for (int c0 = 0; c0 < ds.m; c0 += MCH) {
     const int mc = min(MCH, ds.m - c0);

@@ -33,6 +33,18 @@ struct BatchBase {
   int32_t* status;
   int32_t* counters;  // work-queue counters of the persistent kernels
 };
+// Buffers of a batched FP32-factor / f64-refinement solve: the caller's f64 A
+// and B (B becomes X), the plan's f32 copy of A (then its factor) and f32
+// right-hand side / correction, and f64 copies of B and of the residual.
+struct MixedBase {
+  const double* a64;
+  double* b64;
+  float* a32;
+  float* x32;
+  double* bk64;
+  double* r64;
+  int32_t* status;
+};
 struct BatchItem {
   int64_t a_off;
   int64_t b_off;
@@ -74,7 +86,8 @@ hipError_t launch_potrf_potrs_exact_batched(const SmallDesc<T>* descs, int count
 // taken from a work-queue counter (base.counters[counter]); nmax <= small_max.
 template <typename T>
 hipError_t launch_small_batch(const BatchBase<T>* base, const BatchItem* items, int count, int nmax, int grid,
-                              int counter, hipStream_t s);
+                              int counter, hipStream_t s,
+                              int mode = 1);
 // k_small.hip: grouped diagonal-block factor of step work (one workgroup each).
 template <typename T>
 hipError_t launch_grp_diag(const BatchBase<T>* base, const BatchItem* items, const StepWork* work, int count,
@@ -113,6 +126,14 @@ template <typename T>
 hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha, const T* A, int64_t lda,
                        const T* B, int64_t ldb, T beta, T* C, int64_t ldc, const int32_t* gate,
                        hipStream_t s);
+
+// k_batch.hip: elementwise steps of the batched mixed-precision solve over the
+// packed buffers (op: 0 a32 = a64; 1 x32 = b64, bk64 = b64; 2 b64 = x32;
+// 3 x32 = r64; 4 b64 += x32), and the f64 residual r = bk - A x from the lower
+// triangle of each A, one workgroup per (system, 64-row block).
+hipError_t launch_mixed_flat(const MixedBase* mb, int op, int64_t elems, hipStream_t s);
+hipError_t launch_grp_residual(const MixedBase* mb, const BatchItem* items, const StepWork* work, int count,
+                               hipStream_t s);
 
 // k_gram.hip: C (ka x kb, ldc) = A^T B for tall row-major A (n x ka) and B
 // (n x kb), split over row chunks; partial holds gram_partial_elems(...) elements.

@@ -43,7 +43,8 @@ class RefineInfo(ctypes.Structure):
 
 
 class BatchConfig(ctypes.Structure):
-    _fields_ = [("small_limit", ctypes.c_int32), ("small_waves", ctypes.c_int32)]
+    _fields_ = [("small_limit", ctypes.c_int32), ("small_waves", ctypes.c_int32),
+                ("ir_iterations", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -70,7 +71,7 @@ class ServerStats(ctypes.Structure):
                                                "steps", "bytes_in_use", "bytes_peak", "resident_factors")]
 
 
-PREC = {"f64": 0, "f32": 1}
+PREC = {"f64": 0, "f32": 1, "f32ir": 2}
 FALLBACK_F64 = 1
 
 KERNEL_CLASSES = {"trailing_syrk": 0, "panel_update": 1, "diag_factor": 2, "panel_trsm": 3,
