@@ -336,13 +336,18 @@ def serving_leg(dev, torch, args):
         srv.close()
         lat = np.array([r["latency_s"] for r in res])
         okr = [r for r in res if r["status"] == 0]
+        bad = [(reqs[i][2], reqs[i][4], r["status"], r["failed_row"], r["cache_hit"], r["batched"])
+               for i, r in enumerate(res) if r["status"] != 0]
         span = max(r["done_s"] for r in res) - min(r["submit_s"] for r in res)
         out["rates"].append({"offered_per_s": rate, "submitted": cnt, "achieved_offered_per_s": round(cnt / t_sub, 1),
                              "completed_ok": len(okr), "solves_per_s": round(len(okr) / span, 1),
                              "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                              "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                              "cache_hits": st["cache_hits"], "batched": st["batched"], "steps": st["steps"],
-                             "peak_gb": round(st["bytes_peak"] / 1e9, 2)})
+                             "peak_gb": round(st["bytes_peak"] / 1e9, 2),
+                             "failed": [{"n": int(n_), "precision": p_, "status": int(s_), "failed_row": int(f_),
+                                         "cache_hit": bool(h_), "batched": bool(b_)}
+                                        for n_, p_, s_, f_, h_, b_ in bad[:10]]})
         log(f"bench: serving {out['rates'][-1]}")
         del reqs
     # the plain path: the same kind of requests one at a time (llh_dev_solvec_d on a fresh copy)

@@ -344,7 +344,7 @@ typedef struct llh_result {
   int32_t cache_hit;      /* 1: the resident factor was reused */
   int32_t batched;        /* 1: ran in a step's batched launch */
   int32_t precision_used;
-  int64_t failed_row;     /* NotPositiveDefinite row, else -1 */
+  int64_t failed_row;     /* NotPositiveDefinite row; with LLH_ERR_HIP the hipError_t code; else -1 */
   double submit_s, admit_s, done_s; /* steady-clock seconds */
   double residual;        /* max_j ||b - A x||_inf / (||A||_inf ||x||_inf) for f32 requests, else 0 */
 } llh_result;

@@ -98,6 +98,7 @@ struct Job {
   hipEvent_t done = nullptr;
   size_t reserved = 0;
   StepStage* step = nullptr;  // batched jobs share their step's staging and event
+  hipError_t error = hipSuccess;  // a failed enqueue: reported instead of reading the staging
 };
 
 struct StreamState {
@@ -311,7 +312,8 @@ void finish(llh_server* s, Job& j) {
   Entry* e = j.entry;
   const size_t nm = (size_t)e->n * j.req.m;
   double* hx = static_cast<double*>(j.host);
-  const int32_t info = *reinterpret_cast<int32_t*>(hx + nm + 2 * j.req.m);
+  // a job whose enqueue failed never wrote its staging block
+  const int32_t info = j.error == hipSuccess ? *reinterpret_cast<int32_t*>(hx + nm + 2 * j.req.m) : kNoFail;
   llh_result r{};
   r.cache_hit = j.hit;
   r.batched = j.batched;
@@ -319,7 +321,12 @@ void finish(llh_server* s, Job& j) {
   r.submit_s = j.t_submit;
   r.admit_s = j.t_admit;
   r.failed_row = -1;
-  if (info != kNoFail) {
+  if (j.error != hipSuccess) {
+    r.status = LLH_ERR_HIP;
+    r.failed_row = (int64_t)j.error;  // the HIP error code
+    e->failed = true;                 // its factor may be incomplete: drop the entry when idle
+    set_error(&s->last_error, LLH_ERR_HIP, 0, 0, -1, hipGetErrorString(j.error), (int)j.error);
+  } else if (info != kNoFail) {
     r.status = LLH_ERR_NOT_POSITIVE_DEFINITE;
     r.failed_row = info;
     e->failed = true;
@@ -585,9 +592,24 @@ void worker_loop(llh_server* s) {
       }
       q.done = get_event(s);
       if (x == hipSuccess) x = enqueue_solve(s, q);
-      if (x != hipSuccess) (void)hipEventRecord(q.done, st.s);
+      if (x != hipSuccess) {
+        q.error = x;
+        (void)hipGetLastError();  // clear the sticky launch error before the next enqueue
+        if (!q.hit) {
+          // the factor may be incomplete: no later request may use the entry
+          q.entry->failed = true;
+          (void)hipEventRecord(q.entry->ready, st.s);
+        }
+        (void)hipEventRecord(q.done, st.s);
+      }
     }
     if (!small.empty()) launch_small_step(s, small);
+    if (progressed) {
+      std::lock_guard<std::mutex> g(s->mu);
+      s->stats.bytes_in_use = s->used;
+      s->stats.bytes_peak = s->peak;
+      s->stats.resident_factors = s->cache.size();
+    }
     if (!progressed) {
       std::unique_lock<std::mutex> g(s->mu);
       s->cv_submit.wait_for(g, std::chrono::microseconds(s->jobs.empty() ? 2000 : 20),
@@ -683,10 +705,7 @@ llh_status llh_server_wait(llh_server* s, uint64_t ticket, llh_result* out, llh_
 void llh_server_get_stats(llh_server* s, llh_server_stats* out) {
   if (s == nullptr || s->magic != llh_server::kMagic || out == nullptr) return;
   std::lock_guard<std::mutex> g(s->mu);
-  *out = s->stats;
-  out->bytes_in_use = s->used;
-  out->bytes_peak = s->peak;
-  out->resident_factors = s->cache.size();
+  *out = s->stats;  // the worker publishes the memory figures into stats under mu
 }
 
 void llh_server_evict_all(llh_server* s) {
