@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "linfa-linalg_amd"))
 
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+OUTER = 512  # outer panel width of the library's blocked factor (rank of the trailing SYRK)
 PEAK_HBM_GBS = 8000.0
 
 
@@ -407,13 +408,13 @@ def rocprof_summary(trace_dir, fetch_dir, write_dir, n, out_md, out_json, comman
         counters[ctr] = {"copy_kib": copy, "known_kib": known_kib, "factor": factor, "launches": len(vals),
                          "avg_kib": sum(vals) / len(vals), "bytes_per_launch": sum(vals) / len(vals) * 1024 * factor}
     traffic = counters["FETCH_SIZE"]["bytes_per_launch"] + counters["WRITE_SIZE"]["bytes_per_launch"]
-    # algorithmic bytes of the trailing SYRKs of one factorization (outer block
-    # 256; with lookahead each SYRK leaves out the next panel's 256 columns)
+    # algorithmic bytes of the trailing SYRKs of one factorization (outer panel
+    # width OUTER; with lookahead each SYRK leaves out the next panel's columns)
     alg = []
     k0 = 0
     while k0 < n:
-        kb = min(256, n - k0)
-        m3 = n - k0 - kb - min(256, max(n - k0 - kb, 0))
+        kb = min(OUTER, n - k0)
+        m3 = n - k0 - kb - min(OUTER, max(n - k0 - kb, 0))
         if m3 > 0:
             alg.append(8.0 * (m3 * (m3 + 1) + m3 * kb))
         k0 += kb
@@ -553,8 +554,8 @@ def main():
                                  "a copy of A), %.2fx the algorithmic bytes; profiles/trailing_syrk_f64_n%d.md"
                                  % (tr["fetch"]["factor"], tr["write"]["factor"], tr["traffic_over_algorithmic"], n))
                 if tr else None,
-                "kernel": "gemm_nt_kernel<double,256,128,4,2,TRI=1> (trailing SYRK, rank 256; 128x128 tiles once the "
-                          "trailing matrix is below 4096^2)",
+                "kernel": "gemm_nt_kernel<double,256,128,4,2,TRI,SQ> (trailing SYRK, rank %d; 128x128 tiles once the "
+                          "trailing matrix is below 4096^2)" % OUTER,
                 "avg_launch_us": round(tot_ms / cnt * 1e3, 2), "launches_per_step": cnt // args.steps,
                 "algorithmic_tflop_per_step": round(fl / args.steps / 1e12, 4)}
 
