@@ -6,9 +6,8 @@
 // block in registers.  Inside a panel the column chain uses registers only:
 // the pivot and each L[k, j] come from their lanes by v_readlane.  The panel is
 // then written to a row-major copy of L in LDS, and the rank-8 update of the
-// columns to its right reads L[k, j0 .. j0+8) as four 128-bit uniform-address
-// (broadcast) loads per row k, independent of one another, so they pipeline
-// behind the FMAs.  The inverse runs the same way with lane c holding column c
+// columns to its right reads L[k, j0 .. j0+8) as 128-bit uniform-address
+// (broadcast) loads, software-pipelined behind the FMAs (rank2_rows).  The inverse runs the same way with lane c holding column c
 // of inv(L): inside a panel x_k is scaled by 1/L[k, k] and eliminated from the
 // panel's later rows; then the rows below take the panel's rank-8 update.
 // Each lane loads and stores its own row, all its loads issued together.  Rows
@@ -48,6 +47,35 @@ __device__ __forceinline__ float fma_t(float a, float b, float c) { return __bui
 
 constexpr int kDiagS = kNB + 2;  // LDS row stride (keeps rows 16-byte aligned)
 constexpr int kDiagP = 8;        // panel width
+
+// Applies f(y[k], (col[k*S], col[k*S+1])) to the rows k = K0 .. 63 of the
+// register array y, the column pair read from LDS as one 128-bit uniform load
+// per row.  The rows go in chunks of 8 with the loads of the next chunk issued
+// before the FMAs of the current one (and a scheduling barrier per chunk to
+// keep that order): left alone, the compiler keeps about 3 loads in flight and
+// each row pays most of an LDS round trip.
+template <typename T, int K0, typename F>
+__device__ __forceinline__ void rank2_rows(T* y, const T* col, F&& f) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  constexpr int NC = (kNB - K0) / 8;
+  static_assert((kNB - K0) % 8 == 0, "rows in chunks of 8");
+  if constexpr (NC > 0) {
+    v2 buf[2][8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) buf[0][u] = *reinterpret_cast<const v2*>(&col[(K0 + u) * kDiagS]);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c + 1 < NC) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          buf[(c + 1) & 1][u] = *reinterpret_cast<const v2*>(&col[(K0 + 8 * (c + 1) + u) * kDiagS]);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) f(y[K0 + 8 * c + u], buf[c & 1][u]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
 
 template <typename T>
 struct DiagLds {
@@ -108,12 +136,7 @@ __device__ __forceinline__ void diag_wave_block(TS* __restrict__ a, int64_t lda,
 #pragma unroll
       for (int q = 0; q < P; q += 2) {
         const T p0 = r[j0 + q], p1 = r[j0 + q + 1];
-#pragma unroll
-        for (int k = j0 + P; k < N; ++k) {
-          const v2 lk = *reinterpret_cast<const v2*>(&l[k * S + j0 + q]);
-          r[k] = fma_t(-p0, lk[0], fma_t(-p1, lk[1], r[k]));
-        }
-        __builtin_amdgcn_sched_barrier(0);
+        rank2_rows<T, j0 + P>(r, l + j0 + q, [&](T& y, const v2& lk) { y = fma_t(-p0, lk[0], fma_t(-p1, lk[1], y)); });
       }
     }
   });
@@ -147,12 +170,7 @@ __device__ __forceinline__ void diag_wave_block(TS* __restrict__ a, int64_t lda,
 #pragma unroll
     for (int q = 0; q < P; q += 2) {
       const T p0 = x[k0 + q], p1 = x[k0 + q + 1];
-#pragma unroll
-      for (int ii = k0 + P; ii < N; ++ii) {
-        const v2 li = *reinterpret_cast<const v2*>(&l[ii * S + k0 + q]);
-        x[ii] = fma_t(-li[0], p0, fma_t(-li[1], p1, x[ii]));
-      }
-      __builtin_amdgcn_sched_barrier(0);
+      rank2_rows<T, k0 + P>(x, l + k0 + q, [&](T& y, const v2& li) { y = fma_t(-li[0], p0, fma_t(-li[1], p1, y)); });
     }
   });
 #pragma unroll
