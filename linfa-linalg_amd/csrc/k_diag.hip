@@ -2,19 +2,20 @@
 // (the trailing updates there round differently from the reference anyway, so
 // the block may too; small whole matrices keep the exact-order kernel).
 //
-// One wave does it all, in panels of 8 columns.  Lane i holds row i of the
-// block in registers.  Inside a panel the column chain uses registers only:
-// the pivot and each L[k, j] come from their lanes by v_readlane.  The panel is
-// then written to a row-major copy of L in LDS, and the rank-8 update of the
-// columns to its right reads L[k, j0 .. j0+8) as 128-bit uniform-address
-// (broadcast) loads, software-pipelined behind the FMAs (rank2_rows).  The inverse runs the same way with lane c holding column c
-// of inv(L): inside a panel x_k is scaled by 1/L[k, k] and eliminated from the
-// panel's later rows; then the rows below take the panel's rank-8 update.
-// Each lane loads and stores its own row, all its loads issued together.  Rows
-// beyond b are identity rows, so the block factors to diag(L, I) and its
-// inverse likewise.  Pivot test, failure row and write-back follow
-// cholesky.rs:54-74: on a failure at row f the block holds rows < f and
-// L[f, ..f].
+// Four waves, lane i = row i in every wave; wave w holds columns
+// [16w, 16w + 16) of its row in registers.  The block is factored in panels of
+// 8 columns; panel p belongs to wave p/2, which factors it with the pivot and
+// each L[k, j] taken from their lanes by v_readlane (no barrier inside a panel)
+// and writes it to a row-major copy of L in LDS.  After one barrier every wave
+// applies the panel's rank-8 update to its own columns to the right of the
+// panel, so the trailing work is spread over the four waves.  The inverse is
+// split the same way by rows: lane c holds column c of inv(L), wave w rows
+// [16w, 16w + 16); the wave owning a panel of 8 rows finishes them, publishes
+// them to a double-buffered LDS slab, and after a barrier the waves below take
+// the panel's rank-8 update.  Rows beyond b are identity rows, so the block
+// factors to diag(L, I) and its inverse likewise.  Pivot test, failure row and
+// write-back follow cholesky.rs:54-74: on a failure at row f the block holds
+// rows < f and L[f, ..f].
 #include <type_traits>
 #include <utility>
 
@@ -45,98 +46,98 @@ __device__ __forceinline__ float bcast(float v, int lane) {
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-constexpr int kDiagS = kNB + 2;  // LDS row stride (keeps rows 16-byte aligned)
+constexpr int kDiagS = kNB + 2;  // LDS row stride of L (keeps rows 16-byte aligned)
 constexpr int kDiagP = 8;        // panel width
-
-// Applies f(y[k], (col[k*S], col[k*S+1])) to the rows k = K0 .. 63 of the
-// register array y, the column pair read from LDS as one 128-bit uniform load
-// per row.  The rows go in chunks of 8 with the loads of the next chunk issued
-// before the FMAs of the current one (and a scheduling barrier per chunk to
-// keep that order): left alone, the compiler keeps about 3 loads in flight and
-// each row pays most of an LDS round trip.
-template <typename T, int K0, typename F>
-__device__ __forceinline__ void rank2_rows(T* y, const T* col, F&& f) {
-  typedef T v2 __attribute__((ext_vector_type(2)));
-  constexpr int NC = (kNB - K0) / 8;
-  static_assert((kNB - K0) % 8 == 0, "rows in chunks of 8");
-  if constexpr (NC > 0) {
-    v2 buf[2][8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) buf[0][u] = *reinterpret_cast<const v2*>(&col[(K0 + u) * kDiagS]);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      if (c + 1 < NC) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-          buf[(c + 1) & 1][u] = *reinterpret_cast<const v2*>(&col[(K0 + 8 * (c + 1) + u) * kDiagS]);
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) f(y[K0 + 8 * c + u], buf[c & 1][u]);
-      __builtin_amdgcn_sched_barrier(0);
-    }
-  }
-}
+constexpr int kDiagW = 16;       // columns (factor) / rows (inverse) per wave
+constexpr int kDiagT = 256;      // threads of a diagonal-block workgroup
 
 template <typename T>
 struct DiagLds {
-  T l[kNB * kDiagS];  // L (row-major), written a panel at a time
-  T rinv[kNB];        // 1 / L[j, j]
+  T l[kNB * kDiagS];             // L (row-major), written a panel at a time
+  T xp[2][kDiagP][kNB];          // published rows of inv(L), double-buffered by panel
+  T rinv[kNB];                   // 1 / L[j, j]
+  int fail[kNB / kDiagP];        // per panel: failing column or -1
+  int skip;                      // the system has already failed: nothing to do
 };
 
 // TS: storage type of the block and of the inverse; T: arithmetic type.
-// Launched with exactly one wave.
+// Launched with kDiagT threads.
 template <typename TS, typename T>
-__device__ __forceinline__ void diag_wave_block(TS* __restrict__ a, int64_t lda, int b, TS* __restrict__ inv,
+__device__ __forceinline__ void diag_quad_block(TS* __restrict__ a, int64_t lda, int b, TS* __restrict__ inv,
                                                 int32_t* __restrict__ info, int64_t info_base, DiagLds<T>& sm) {
-  constexpr int N = kNB, S = kDiagS, P = kDiagP;
+  constexpr int N = kNB, S = kDiagS, P = kDiagP, WC = kDiagW;
   typedef T v2 __attribute__((ext_vector_type(2)));
-  const int i = threadIdx.x;  // this lane's row (factor) / column (loads, stores, inverse)
-  T* l = sm.l;  // row-major L, published a panel at a time
-  // lane i loads its own row; masked entries read the block's first element
-  // (always in bounds) so that the 64 loads carry no branches and issue together
-  T r[N];
+  const int i = threadIdx.x & 63;     // row (factor) / column of inv(L) (inverse)
+  const int w = threadIdx.x >> 6;     // wave: columns (factor) / rows (inverse) [16w, 16w + 16)
+  const int c0 = w * WC;
+  T* l = sm.l;
+  if (threadIdx.x < N / P) sm.fail[threadIdx.x] = -1;
+  // this lane's 16 columns of row i; masked entries read the block's first
+  // element (always in bounds) so that the loads carry no branches
+  T r[WC];
   {
     const TS* row = a + (int64_t)min(i, b - 1) * lda;
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-      const bool in = i < b && j <= i;
-      const T v = T(*(in ? row + j : a));
-      r[j] = in ? v : ((i >= b && i == j) ? T(1) : T(0));
+    for (int kk = 0; kk < WC; ++kk) {
+      const int k = c0 + kk;
+      const bool in = i < b && k <= i;
+      const T v = T(*(in ? row + k : a));
+      r[kk] = in ? v : ((i >= b && i == k) ? T(1) : T(0));
     }
   }
+  __syncthreads();
   // (no early exit: a failure only turns the remaining steps off, so every loop
   // keeps its constant trip count and r stays in registers)
   int fail = -1;
   static_for(std::make_integer_sequence<int, N / P>{}, [&](auto pi) {
-    constexpr int j0 = decltype(pi)::value * P;
+    constexpr int p = decltype(pi)::value, j0 = p * P, lo = j0 % WC;
+    if (w == p / 2 && fail < 0) {
+      int f = -1;
 #pragma unroll
-    for (int c = 0; c < P; ++c) {
-      const int j = j0 + c;
-      if (fail < 0) {
-        const T d = bcast(r[j], j);
-        if (d <= T(0)) {  // cholesky.rs:69 (NaN passes)
-          fail = j;
-        } else {
-          const T piv = sqrt_rn(d);
-          const T rp = T(1) / piv;
-          r[j] = (i == j) ? piv : r[j] * rp;
-          if (i == 0) sm.rinv[j] = rp;
+      for (int c = 0; c < P; ++c) {
+        const int j = j0 + c;
+        if (f < 0) {
+          const T d = bcast(r[lo + c], j);
+          if (d <= T(0)) {  // cholesky.rs:69 (NaN passes)
+            f = j;
+          } else {
+            const T piv = sqrt_rn(d);
+            const T rp = T(1) / piv;
+            r[lo + c] = (i == j) ? piv : r[lo + c] * rp;
+            if (i == 0) sm.rinv[j] = rp;
 #pragma unroll
-          for (int q = c + 1; q < P; ++q) r[j0 + q] -= r[j] * bcast(r[j], j0 + q);
+            for (int q = c + 1; q < P; ++q) r[lo + q] -= r[lo + c] * bcast(r[lo + c], j0 + q);
+          }
         }
       }
-    }
-    if (fail < 0) {
-      // publish the panel (this lane's row), then the rank-8 update to its right
 #pragma unroll
-      for (int q = 0; q < P; q += 2) *reinterpret_cast<v2*>(&l[i * S + j0 + q]) = v2{r[j0 + q], r[j0 + q + 1]};
-      __builtin_amdgcn_wave_barrier();
-      // (panel column pairs outermost: the pair stays in registers while the
-      // rows stream past, which keeps the register count bounded)
+      for (int q = 0; q < P; q += 2) *reinterpret_cast<v2*>(&l[i * S + j0 + q]) = v2{r[lo + q], r[lo + q + 1]};
+      if (i == 0) sm.fail[p] = f;
+    }
+    __syncthreads();
+    if (fail < 0) fail = sm.fail[p];
+    if (fail < 0 && c0 + WC > j0 + P) {
+      // rank-8 update of this wave's columns right of the panel:
+      // r[k] -= sum_q L[i, j0+q] L[k, j0+q]
+      T pr[P];
 #pragma unroll
       for (int q = 0; q < P; q += 2) {
-        const T p0 = r[j0 + q], p1 = r[j0 + q + 1];
-        rank2_rows<T, j0 + P>(r, l + j0 + q, [&](T& y, const v2& lk) { y = fma_t(-p0, lk[0], fma_t(-p1, lk[1], y)); });
+        const v2 v = *reinterpret_cast<const v2*>(&l[i * S + j0 + q]);
+        pr[q] = v[0];
+        pr[q + 1] = v[1];
+      }
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk) {
+        const int k = c0 + kk;
+        if (k >= j0 + P) {
+          T t = r[kk];
+#pragma unroll
+          for (int q = 0; q < P; q += 2) {
+            const v2 lk = *reinterpret_cast<const v2*>(&l[k * S + j0 + q]);
+            t = fma_t(-pr[q], lk[0], fma_t(-pr[q + 1], lk[1], t));
+          }
+          r[kk] = t;
+        }
       }
     }
   });
@@ -144,52 +145,73 @@ __device__ __forceinline__ void diag_wave_block(TS* __restrict__ a, int64_t lda,
   // row f up to f-1
   {
     const int last = (fail < 0 || i < fail) ? i : (i == fail ? i - 1 : -1);
-    TS* row = a + (int64_t)i * lda;
+    TS* row = a + (int64_t)i * lda + c0;
 #pragma unroll
-    for (int j = 0; j < N; ++j)
-      if (i < b && j <= last) row[j] = TS(r[j]);
+    for (int kk = 0; kk < WC; ++kk)
+      if (i < b && c0 + kk <= last) row[kk] = TS(r[kk]);
   }
   if (fail >= 0) {
-    if (i == 0) atomicMin(info, (int32_t)(info_base + fail));
+    if (threadIdx.x == 0) atomicMin(info, (int32_t)(info_base + fail));
     return;
   }
-  // inv(L): lane c solves L x = e_c in panels of 8
+  // inv(L): lane c solves L x = e_c; this wave holds rows [c0, c0 + 16) of x
   const int c = i;
-  T x[N];
+  T x[WC];
 #pragma unroll
-  for (int k = 0; k < N; ++k) x[k] = (k == c) ? T(1) : T(0);
+  for (int kk = 0; kk < WC; ++kk) x[kk] = (c0 + kk == c) ? T(1) : T(0);
   static_for(std::make_integer_sequence<int, N / P>{}, [&](auto pi) {
-    constexpr int k0 = decltype(pi)::value * P;
+    constexpr int p = decltype(pi)::value, k0 = p * P, lo = k0 % WC;
+    if (w == p / 2) {
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-      const int k = k0 + q;
-      x[k] *= sm.rinv[k];
+      for (int q = 0; q < P; ++q) {
+        const int k = k0 + q;
+        x[lo + q] *= sm.rinv[k];
 #pragma unroll
-      for (int q2 = q + 1; q2 < P; ++q2) x[k0 + q2] -= l[(k0 + q2) * S + k] * x[k];
+        for (int q2 = q + 1; q2 < P; ++q2) x[lo + q2] -= l[(k0 + q2) * S + k] * x[lo + q];
+      }
+#pragma unroll
+      for (int q = 0; q < P; ++q) sm.xp[p & 1][q][c] = x[lo + q];
     }
+    __syncthreads();
+    if (c0 + WC > k0 + P) {
+      T xq[P];
 #pragma unroll
-    for (int q = 0; q < P; q += 2) {
-      const T p0 = x[k0 + q], p1 = x[k0 + q + 1];
-      rank2_rows<T, k0 + P>(x, l + k0 + q, [&](T& y, const v2& li) { y = fma_t(-li[0], p0, fma_t(-li[1], p1, y)); });
+      for (int q = 0; q < P; ++q) xq[q] = sm.xp[p & 1][q][c];
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk) {
+        const int ii = c0 + kk;
+        if (ii >= k0 + P) {
+          T t = x[kk];
+#pragma unroll
+          for (int q = 0; q < P; q += 2) {
+            const v2 li = *reinterpret_cast<const v2*>(&l[ii * S + k0 + q]);
+            t = fma_t(-li[0], xq[q], fma_t(-li[1], xq[q + 1], t));
+          }
+          x[kk] = t;
+        }
+      }
     }
   });
 #pragma unroll
-  for (int ii = 0; ii < N; ++ii) inv[ii * N + c] = TS(x[ii]);
+  for (int kk = 0; kk < WC; ++kk) inv[(c0 + kk) * N + c] = TS(x[kk]);
 }
 
 template <typename TS, typename T>
-__global__ __launch_bounds__(64) void diag_fast_kernel(TS* __restrict__ a, int64_t lda, int b,
-                                                       TS* __restrict__ inv, int32_t* __restrict__ info,
-                                                       int64_t info_base, const int32_t* __restrict__ gate) {
-  if (gate != nullptr && *gate != kNoFail) return;
+__global__ __launch_bounds__(kDiagT) void diag_fast_kernel(TS* __restrict__ a, int64_t lda, int b,
+                                                           TS* __restrict__ inv, int32_t* __restrict__ info,
+                                                           int64_t info_base, const int32_t* __restrict__ gate) {
   __shared__ __attribute__((aligned(16))) DiagLds<T> sm;
-  diag_wave_block<TS, T>(a, lda, b, inv, info, info_base, sm);
+  // the gate is read once and shared, so all four waves leave or stay together
+  if (threadIdx.x == 0) sm.skip = gate != nullptr && *gate != kNoFail;
+  __syncthreads();
+  if (sm.skip) return;
+  diag_quad_block<TS, T>(a, lda, b, inv, info, info_base, sm);
 }
 
 // Grouped form for the batch planner: work w = (item, k0) factors and inverts
 // the block at (k0, k0) of its system; a failure sets the system's status.
 template <typename T>
-__global__ __launch_bounds__(64) void grp_diag_fast_kernel(const BatchBase<T>* __restrict__ base,
+__global__ __launch_bounds__(kDiagT) void grp_diag_fast_kernel(const BatchBase<T>* __restrict__ base,
                                                            const BatchItem* __restrict__ items,
                                                            const StepWork* __restrict__ work, int count) {
   if ((int)blockIdx.x >= count) return;
@@ -197,9 +219,11 @@ __global__ __launch_bounds__(64) void grp_diag_fast_kernel(const BatchBase<T>* _
   const BatchItem it = items[w.item];
   const BatchBase<T> bb = *base;
   int32_t* st = bb.status + it.id;
-  if (*st != kNoFail) return;
   __shared__ __attribute__((aligned(16))) DiagLds<T> sm;
-  diag_wave_block<T, T>(bb.a + it.a_off + (int64_t)w.k0 * it.lda + w.k0, it.lda, min(kNB, it.n - w.k0),
+  if (threadIdx.x == 0) sm.skip = *st != kNoFail;
+  __syncthreads();
+  if (sm.skip) return;
+  diag_quad_block<T, T>(bb.a + it.a_off + (int64_t)w.k0 * it.lda + w.k0, it.lda, min(kNB, it.n - w.k0),
                         bb.linv + it.linv_off + (int64_t)(w.k0 / kNB) * kNB * kNB, st, w.k0, sm);
 }
 
@@ -209,7 +233,7 @@ template <typename T>
 hipError_t launch_grp_diag_fast(const BatchBase<T>* base, const BatchItem* items, const StepWork* work,
                                 int count, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL((grp_diag_fast_kernel<T>), dim3(count), dim3(64), 0, s, base, items, work, count);
+  hipLaunchKernelGGL((grp_diag_fast_kernel<T>), dim3(count), dim3(kDiagT), 0, s, base, items, work, count);
   return hipGetLastError();
 }
 template hipError_t launch_grp_diag_fast<double>(const BatchBase<double>*, const BatchItem*, const StepWork*,
@@ -222,7 +246,8 @@ hipError_t launch_diag_fast(TS* a, int64_t lda, int b, TS* inv, int32_t* info, i
                             const int32_t* gate, hipStream_t s) {
   if (b <= 0) return hipSuccess;
   if (b > kNB) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((diag_fast_kernel<TS, T>), dim3(1), dim3(64), 0, s, a, lda, b, inv, info, info_base, gate);
+  hipLaunchKernelGGL((diag_fast_kernel<TS, T>), dim3(1), dim3(kDiagT), 0, s, a, lda, b, inv, info, info_base,
+                     gate);
   return hipGetLastError();
 }
 
