@@ -200,16 +200,22 @@ LLH_DEV_DECL(float, s)
  * f64, and solves refine x in f64 against the caller's f64 A:
  *   x = solve_low(b); repeat { r = b - A x (f64); stop when every column has
  *   ||r||_inf <= ||x||_inf * ||A||_inf * eps64 * sqrt(n); x += solve_low(r) }.
+ * (LLH_PREC_F16 applies the stopping test to the scaled system it factors.)
  * A is read from its lower triangle only and must stay valid and unchanged
  * while the factor is used.  A factor is bound to its context's stream.
  * ------------------------------------------------------------------------- */
 typedef enum llh_precision {
-  LLH_PREC_F64 = 0,         /* plain f64 factor, no refinement */
-  LLH_PREC_F32 = 1,         /* f32 trailing updates on MFMA, f64 diagonal blocks, f64 refinement */
-  LLH_PREC_F32_REFINED = 2  /* batch plans only: f64 buffers, f32 factor and solves, f64 refinement */
+  LLH_PREC_F64 = 0,          /* plain f64 factor, no refinement */
+  LLH_PREC_F32 = 1,          /* f32 trailing updates on MFMA, f64 diagonal blocks, f64 refinement */
+  LLH_PREC_F32_REFINED = 2,  /* batch plans only: f64 buffers, f32 factor and solves, f64 refinement */
+  LLH_PREC_F16 = 3           /* factor and solve only: as LLH_PREC_F32, but of diag(s) A diag(s) with
+                                s = diag(A)^-1/2, and each trailing SYRK multiplies f16 panels with f32
+                                accumulation.  Refinement contracts the error by about cond(A) * 2^-11
+                                per step, so it suits cond(A) up to ~1e2..1e3; a step that fails to
+                                halve the residual ends the refinement as not converged. */
 } llh_precision;
 
-enum { LLH_FACTOR_FALLBACK_F64 = 1 }; /* on f32 breakdown (pivot <= 0) or stalled refinement, redo in f64 */
+enum { LLH_FACTOR_FALLBACK_F64 = 1 }; /* on low-precision breakdown (pivot <= 0) or stalled refinement, redo in f64 */
 
 typedef struct llh_refine_opts {
   int32_t max_iter; /* corrections before giving up (default 30 when <= 0) */
@@ -221,16 +227,17 @@ typedef struct llh_refine_info {
   int32_t converged;      /* 1 when the stopping test passed */
   int32_t fell_back;      /* 1 when the f64 factor produced the result */
   int32_t precision_used; /* llh_precision of the factor that produced x */
-  double residual;        /* max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf) */
-  double anorm;           /* ||A||_inf */
+  double residual;        /* max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf); for
+                             LLH_PREC_F16 of the scaled system S A S (x / s) = S b, S = diag(s) */
+  double anorm;           /* ||A||_inf (LLH_PREC_F16: ||S A S||_inf) */
 } llh_refine_info;
 
 typedef struct llh_factor llh_factor;
 
 /* Factors the device f64 matrix a (n x n, row-major lda, lower triangle read).
- * With LLH_PREC_F32 and LLH_FACTOR_FALLBACK_F64 in flags, a pivot failure of the
- * f32 factor falls back to an f64 factor; otherwise it reports
- * NotPositiveDefinite with the failing row. */
+ * With LLH_PREC_F32 or LLH_PREC_F16 and LLH_FACTOR_FALLBACK_F64 in flags, a pivot
+ * failure of the low-precision factor falls back to an f64 factor; otherwise it
+ * reports NotPositiveDefinite with the failing row. */
 llh_status llh_factor_create(llh_context* ctx, llh_precision prec, size_t n, const double* a, size_t lda,
                              int32_t flags, llh_factor** out, llh_error* err);
 /* x (n x m, ldx) = A^-1 b (n x m, ldb), device f64; b and x may not alias. */

@@ -666,6 +666,8 @@ void llh_context_destroy(llh_context* ctx) {
   if (ctx->dinv.ptr) (void)hipFree(ctx->dinv.ptr);
   if (ctx->mixed_l.ptr) (void)hipFree(ctx->mixed_l.ptr);
   if (ctx->mixed_work.ptr) (void)hipFree(ctx->mixed_work.ptr);
+  if (ctx->mixed_scale.ptr) (void)hipFree(ctx->mixed_scale.ptr);
+  if (ctx->half_panel.ptr) (void)hipFree(ctx->half_panel.ptr);
   if (ctx->sync.ptr) (void)hipFree(ctx->sync.ptr);
   if (ctx->d_info) (void)hipFree(ctx->d_info);
   if (ctx->h_info) (void)hipHostFree(ctx->h_info);

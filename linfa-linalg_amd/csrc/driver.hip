@@ -65,6 +65,21 @@ hipError_t factor_panel(T* a, int64_t lda, int n, int k0, int kb, int32_t* info,
   return hipSuccess;
 }
 
+// The trailing update C -= P P^T (lower triangle of the m x m C, P m x kb).
+// With half (f32 storage only), P is first rounded to f16 into half (ld kb) and
+// the SYRK multiplies in f16 with f32 accumulation.
+template <typename T>
+hipError_t trailing_syrk(int m, int kb, const T* p, int64_t lda, T* c, int32_t* info, _Float16* half,
+                         hipStream_t s) {
+  if constexpr (sizeof(T) == 4) {
+    if (half != nullptr) {
+      LLH_TRY(launch_convert_s2h(m, kb, p, lda, half, kb, s));
+      return launch_syrk_lower_h(m, kb, -1.0f, half, kb, 1.0f, c, lda, info, s);
+    }
+  }
+  return launch_syrk_lower<T>(m, kb, T(-1), p, lda, T(1), c, lda, info, s);
+}
+
 }  // namespace
 
 // Right-looking blocked Cholesky of the lower triangle (the blocked equivalent
@@ -77,8 +92,12 @@ hipError_t factor_panel(T* a, int64_t lda, int n, int k0, int kb, int32_t* info,
 // operation order.
 template <typename T>
 hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, T* dinv, hipStream_t s,
-                         Profiler* prof, bool diag_f64, const Lookahead* la) {
+                         Profiler* prof, bool diag_f64, const Lookahead* la, _Float16* half) {
   const double es = (double)sizeof(T);
+  if (half != nullptr && sizeof(T) != 4) return hipErrorInvalidValue;
+  // bytes per panel element of the trailing update (the f16 copy is written
+  // and read once more)
+  const double pes = half != nullptr ? es + 4.0 : es;
   LLH_TRY(launch_fill_i32(info, kNoFail, 1, s));
   if (n <= 0) return hipSuccess;
   if (n <= small_max<T>() || dinv == nullptr) {
@@ -99,8 +118,8 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
       // triangle of C once, read the panel once.
       if (la == nullptr) {
         const double outs = (double)m2 * (m2 + 1) / 2.0;
-        LLH_PROF(kTrailingSyrk, 2.0 * outs * kb, es * (2.0 * outs + (double)m2 * kb),
-                 (launch_syrk_lower<T>(m2, kb, T(-1), p, lda, T(1), c, lda, info, s)));
+        LLH_PROF(kTrailingSyrk, 2.0 * outs * kb, es * 2.0 * outs + pes * (double)m2 * kb,
+                 (trailing_syrk<T>(m2, kb, p, lda, c, info, half, s)));
         LLH_TRY(factor_panel<T>(a, lda, n, k0 + kb, kb2, info, dinv, s, prof, diag_f64));
         continue;
       }
@@ -120,8 +139,8 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
       if (m3 > 0) {
         const double outs = (double)m3 * (m3 + 1) / 2.0;
         T* p3 = p + (int64_t)kb2 * lda;
-        LLH_PROF(kTrailingSyrk, 2.0 * outs * kb, es * (2.0 * outs + (double)m3 * kb),
-                 (launch_syrk_lower<T>(m3, kb, T(-1), p3, lda, T(1), c + (int64_t)kb2 * lda + kb2, lda, info, s)));
+        LLH_PROF(kTrailingSyrk, 2.0 * outs * kb, es * 2.0 * outs + pes * (double)m3 * kb,
+                 (trailing_syrk<T>(m3, kb, p3, lda, c + (int64_t)kb2 * lda + kb2, info, half, s)));
       }
       LLH_TRY(hipStreamWaitEvent(s, la->join, 0));
     }
@@ -190,9 +209,9 @@ hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t ld
 }
 
 template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool, double*, hipStream_t,
-                                          Profiler*, bool, const Lookahead*);
+                                          Profiler*, bool, const Lookahead*, _Float16*);
 template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, float*, hipStream_t, Profiler*,
-                                         bool, const Lookahead*);
+                                         bool, const Lookahead*, _Float16*);
 template hipError_t trsm_enqueue<double>(bool, bool, int, const double*, int64_t, int, double*, int64_t,
                                          const double*, bool, double*, SolveSync*, hipStream_t, Profiler*);
 template hipError_t trsm_enqueue<float>(bool, bool, int, const float*, int64_t, int, float*, int64_t,

@@ -210,10 +210,11 @@ __device__ inline void tri_tile(int lin, int tiles_m, int tiles_n, int R, int& t
 }
 
 // SQ: the square triangular case (TRI and M == N), i.e. the trailing SYRK.
-template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool SQ, bool VEC>
+// TI: operand type (T, or _Float16 for the f32 SYRK on f16 operands).
+template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool SQ, bool VEC, typename TI = T>
 __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_kernel(int M, int N, int K, T alpha,
-                                                                 const T* A, int64_t lda,
-                                                                 const T* __restrict__ B, int64_t ldb, T beta,
+                                                                 const TI* A, int64_t lda,
+                                                                 const TI* __restrict__ B, int64_t ldb, T beta,
                                                                  T* C, int64_t ldc,
                                                                  const int32_t* __restrict__ gate, int tiles_m,
                                                                  int tiles_n, int total) {
@@ -242,29 +243,30 @@ __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_ker
   if (TRI && m0 + BM - 1 < n0) return;
 
   // C prefetch for the f64 trailing SYRK on 256 x 128 tiles (measured: it
-  // slows the f32 tiles, and the 128 x 128 f64 tile has no registers left)
-  constexpr bool PRE = SQ && sizeof(T) == 8 && BM == 256;
-  nt_tile<T, BM, BN, WM, WN, TRI, VEC, PRE>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
+  // slows the f32 and f16 tiles, and the 128 x 128 f64 tile has no registers
+  // left)
+  constexpr bool PRE = SQ && BM == 256 && sizeof(T) == 8;
+  nt_tile<T, BM, BN, WM, WN, TRI, VEC, PRE, TI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
 }
 
 // SQ launches the square lower-triangle form (M == N, TRI), which only the
 // trailing SYRK uses (launch_syrk_lower), so that it is its own instantiation.
-template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool SQ = false>
-hipError_t launch_nt(int M, int N, int K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
+template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool SQ = false, typename TI = T>
+hipError_t launch_nt(int M, int N, int K, T alpha, const TI* A, int64_t lda, const TI* B, int64_t ldb, T beta,
                      T* C, int64_t ldc, const int32_t* gate, hipStream_t s) {
   static_assert(!SQ || TRI, "the square form is triangular");
   if (SQ && M != N) return hipErrorInvalidValue;
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   const int total = SQ ? tri_total(tiles_m, tiles_n, BM / BN) : tiles_m * tiles_n;
   const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
-                   ((lda * sizeof(T)) % 16 == 0) && ((ldb * sizeof(T)) % 16 == 0);
+                   ((lda * sizeof(TI)) % 16 == 0) && ((ldb * sizeof(TI)) % 16 == 0);
   // The kernel deals WG ids to XCDs as lin = (id % 8) * per + id / 8, which is a
   // bijection onto [0, 8 * per) only when the grid has exactly 8 * per WGs.
   const int per = (total + 7) / 8;
   const dim3 grid(8 * per), block(WM * WN * 64);
-#define LLH_NT_GO(SQ, VEC)                                                                                  \
-  hipLaunchKernelGGL((gemm_nt_kernel<T, BM, BN, WM, WN, TRI, SQ, VEC>), grid, block, 0, s, M, N, K, alpha, A, lda, \
-                     B, ldb, beta, C, ldc, gate, tiles_m, tiles_n, total)
+#define LLH_NT_GO(SQ, VEC)                                                                                    \
+  hipLaunchKernelGGL((gemm_nt_kernel<T, BM, BN, WM, WN, TRI, SQ, VEC, TI>), grid, block, 0, s, M, N, K, alpha, \
+                     A, lda, B, ldb, beta, C, ldc, gate, tiles_m, tiles_n, total)
   if (vec) LLH_NT_GO(SQ, true);
   else LLH_NT_GO(SQ, false);
 #undef LLH_NT_GO
@@ -341,6 +343,20 @@ hipError_t launch_syrk_lower(int n, int k, T alpha, const T* A, int64_t lda, T b
   if (sizeof(T) == 8 && (int64_t)n * n >= (int64_t)4096 * 4096)
     return launch_nt<T, 256, 128, 4, 2, true, true>(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, gate, s);
   return launch_nt<T, 128, 128, 2, 2, true, true>(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, gate, s);
+}
+
+hipError_t launch_syrk_lower_h(int n, int k, float alpha, const _Float16* A, int64_t lda, float beta, float* C,
+                               int64_t ldc, const int32_t* gate, hipStream_t s, int variant) {
+  if (n <= 0) return hipSuccess;
+  if (k < 0) k = 0;
+  // 128 x 128 tiles: two workgroups per CU (72 KB of LDS each) hide more of the
+  // operand latency than one 256 x 128 workgroup, measured 200 against 168
+  // TFLOP/s at n = 32256, k = 512
+  if (variant == 1)
+    return launch_nt<float, 256, 128, 4, 2, true, true, _Float16>(n, n, k, alpha, A, lda, A, lda, beta, C, ldc,
+                                                                  gate, s);
+  return launch_nt<float, 128, 128, 2, 2, true, true, _Float16>(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, gate,
+                                                                s);
 }
 
 template hipError_t launch_syrk_lower<double>(int, int, double, const double*, int64_t, double, double*, int64_t,

@@ -7,6 +7,10 @@
 //   A operand: lane l holds op(A)[l%16][k0 + l/16]
 //   B operand: lane l holds op(B)[k0 + l/16][l%16]
 //   f64 result reg i: C[4*i + l/16][l%16];  f32 result reg i: C[4*(l/16) + i][l%16]
+// The f16 form (v_mfma_f32_16x16x32_f16) takes 8 consecutive k per lane,
+// op(A)[l%16][k0 + 8*(l/16) + 0..7] and likewise for B, and returns the f32
+// layout.  A sum over k does not depend on which k a lane group holds, as long
+// as A and B are fed the same way, which the tile below guarantees.
 #pragma once
 
 #include "llh_common.h"
@@ -36,6 +40,20 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
   static constexpr int kRowStep = 1;
 };
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+template <>
+struct Mfma<_Float16> {
+  using acc_t = v4f;
+  static __device__ __forceinline__ acc_t mma(v8h a, v8h b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
+  static constexpr int kRowStep = 1;
+};
+
+// Operand elements one lane feeds into one MFMA: 1 for f64 / f32, 8 for f16.
+template <typename TI>
+constexpr int mfma_lane_k() { return sizeof(TI) == 2 ? 8 : 1; }
 
 // One BM x BN tile of C = beta*C + alpha*A*B^T on WM x WN waves.
 //
@@ -59,27 +77,33 @@ struct Mfma<float> {
 // acc += (beta/alpha) C, so the epilogue only stores alpha * acc.  Without it
 // every workgroup of a rank-k update with small k reads its C tile at the same
 // moment, after its MFMAs, and the launch waits on one HBM burst per tile.
-template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool VEC, bool PRE = false>
-__device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const T* A, int64_t lda,
-                                        const T* __restrict__ B, int64_t ldb, T beta, T* C, int64_t ldc,
+//
+// TI is the operand type: T itself, or _Float16 with T = float (f16 products
+// accumulated in f32; one MFMA then takes a lane's whole 8-element fragment).
+template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool VEC, bool PRE = false, typename TI = T>
+__device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const TI* A, int64_t lda,
+                                        const TI* __restrict__ B, int64_t ldb, T beta, T* C, int64_t ldc,
                                         int m0, int n0) {
   constexpr int NTHR = WM * WN * 64;
   constexpr int TM = BM / WM, TN = BN / WN;
   constexpr int FM = TM / 16, FN = TN / 16;
-  constexpr int KPL = 16 / sizeof(T);             // k per lane per 128-bit read
-  constexpr int SUB = 4 * KPL;                    // k covered by KPL MFMAs
+  constexpr int KPL = 16 / sizeof(TI);            // k per lane per 128-bit read
+  constexpr int OPK = mfma_lane_k<TI>();          // of which one MFMA takes OPK
+  static_assert(OPK == 1 || OPK == KPL, "an MFMA takes one element or the whole fragment");
+  static_assert(sizeof(TI) == sizeof(T) || (sizeof(TI) == 2 && sizeof(T) == 4), "f16 operands accumulate in f32");
+  constexpr int SUB = 4 * KPL;                    // k covered by KPL / OPK MFMAs
   constexpr int SK = 2 * SUB;                     // k per slab
   constexpr int LDK = SK + KPL;                   // padded LDS row stride (elements)
   constexpr int LPR = SK / KPL;                   // lanes per slab row
   constexpr int RPP = NTHR / LPR;                 // rows per load pass
   constexpr int PA = BM / RPP, PB = BN / RPP;     // load passes
   static_assert(BM % RPP == 0 && BN % RPP == 0, "tile rows must divide the load passes");
-  using MF = Mfma<T>;
+  using MF = Mfma<TI>;
   using acc_t = typename MF::acc_t;
-  typedef T vec_t __attribute__((ext_vector_type(KPL)));
+  typedef TI vec_t __attribute__((ext_vector_type(KPL)));
 
-  __shared__ __attribute__((aligned(16))) T As[2][BM * LDK];
-  __shared__ __attribute__((aligned(16))) T Bs[2][BN * LDK];
+  __shared__ __attribute__((aligned(16))) TI As[2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) TI Bs[2][BN * LDK];
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -87,8 +111,8 @@ __device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const T* A
   const int lrow = t / LPR, lk = (t % LPR) * KPL;
 
   // Clamped source rows of this thread's loads.
-  const T* pa[PA];
-  const T* pb[PB];
+  const TI* pa[PA];
+  const TI* pb[PB];
 #pragma unroll
   for (int p = 0; p < PA; ++p) pa[p] = A + (int64_t)min(m0 + lrow + p * RPP, M - 1) * lda + lk;
 #pragma unroll
@@ -111,11 +135,11 @@ __device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const T* A
 #pragma unroll
       for (int p = 0; p < PA; ++p)
 #pragma unroll
-        for (int e = 0; e < KPL; ++e) ra[p][e] = (k0 + lk + e < K) ? pa[p][k0 + e] : T(0);
+        for (int e = 0; e < KPL; ++e) ra[p][e] = (k0 + lk + e < K) ? pa[p][k0 + e] : TI(0);
 #pragma unroll
       for (int p = 0; p < PB; ++p)
 #pragma unroll
-        for (int e = 0; e < KPL; ++e) rb[p][e] = (k0 + lk + e < K) ? pb[p][k0 + e] : T(0);
+        for (int e = 0; e < KPL; ++e) rb[p][e] = (k0 + lk + e < K) ? pb[p][k0 + e] : TI(0);
     }
   };
   auto store = [&](int buf) {
@@ -166,8 +190,8 @@ __device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const T* A
     if (PRE && pre && sl > pre0) cadd(sl - 1 - pre0);
     if (more) load((sl + 1) * SK);
     if (PRE && pre && sl >= pre0) cload(sl - pre0);
-    const T* as = As[cur];
-    const T* bs = Bs[cur];
+    const TI* as = As[cur];
+    const TI* bs = Bs[cur];
 #pragma unroll
     for (int c = 0; c < SK; c += SUB) {
       vec_t af[FM], bf[FN];
@@ -175,12 +199,19 @@ __device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const T* A
       for (int i = 0; i < FM; ++i) af[i] = *reinterpret_cast<const vec_t*>(&as[(wm0 + i * 16 + fr) * LDK + c + fq]);
 #pragma unroll
       for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const vec_t*>(&bs[(wn0 + j * 16 + fr) * LDK + c + fq]);
+      if constexpr (OPK == 1) {
 #pragma unroll
-      for (int s = 0; s < KPL; ++s)
+        for (int s = 0; s < KPL; ++s)
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[i][s], bf[j][s], acc[i][j]);
+      } else {
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[i][s], bf[j][s], acc[i][j]);
+          for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(af[i], bf[j], acc[i][j]);
+      }
     }
     if (more) store(cur ^ 1);
     __syncthreads();

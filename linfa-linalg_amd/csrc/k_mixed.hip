@@ -8,25 +8,68 @@
 namespace llh {
 namespace {
 
-// Y (rows x cols, f32) = X (f64); lower_only restricts to j <= i.
+// Y (rows x cols, f32) = rs[r] * X (f64) * cs[c]; lower_only restricts to
+// j <= i; a null scale is 1.
 __global__ void convert_d2s_kernel(int rows, int cols, const double* __restrict__ X, int64_t ldx,
-                                   float* __restrict__ Y, int64_t ldy, bool lower_only) {
+                                   float* __restrict__ Y, int64_t ldy, bool lower_only,
+                                   const double* __restrict__ rs, const double* __restrict__ cs) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y + blockIdx.z * 65535;
   if (r >= rows || c >= cols || (lower_only && c > r)) return;
-  Y[(int64_t)r * ldy + c] = (float)X[(int64_t)r * ldx + c];
+  double x = X[(int64_t)r * ldx + c];
+  if (rs != nullptr) x *= rs[r];
+  if (cs != nullptr) x *= cs[c];
+  Y[(int64_t)r * ldy + c] = (float)x;
 }
 
-// X (f64) += D (f32)
+// X (f64) += rs[r] * D (f32); a null scale is 1.
 __global__ void axpy_s2d_kernel(int rows, int cols, const float* __restrict__ D, int64_t ldd,
-                                double* __restrict__ X, int64_t ldx) {
+                                double* __restrict__ X, int64_t ldx, const double* __restrict__ rs) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y + blockIdx.z * 65535;
   if (r >= rows || c >= cols) return;
-  X[(int64_t)r * ldx + c] += (double)D[(int64_t)r * ldd + c];
+  double d = (double)D[(int64_t)r * ldd + c];
+  if (rs != nullptr) d *= rs[r];
+  X[(int64_t)r * ldx + c] += d;
 }
 
-// R = B - A X (ABS: R = |A| 1, the row sums of |A|), A symmetric from its lower
+// Y (f16) = X (f32), rounded to nearest even, 8 columns per thread when the
+// rows are 16-byte aligned.
+template <bool VEC>
+__global__ void convert_s2h_kernel(int rows, int cols, const float* __restrict__ X, int64_t ldx,
+                                   _Float16* __restrict__ Y, int64_t ldy) {
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) * 8;
+  const int r = blockIdx.y + blockIdx.z * 65535;
+  if (r >= rows || c >= cols) return;
+  const float* x = X + (int64_t)r * ldx + c;
+  _Float16* y = Y + (int64_t)r * ldy + c;
+  if (VEC && c + 8 <= cols) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+    const v4 a = *reinterpret_cast<const v4*>(x), b = *reinterpret_cast<const v4*>(x + 4);
+    h8 h;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      h[e] = (_Float16)a[e];
+      h[e + 4] = (_Float16)b[e];
+    }
+    *reinterpret_cast<h8*>(y) = h;
+  } else {
+    for (int e = 0; e < 8 && c + e < cols; ++e) y[e] = (_Float16)x[e];
+  }
+}
+
+// s[i] = 1 / sqrt(A[i][i]) where the diagonal is positive and finite, else 1
+// (the factor then meets the same non-positive pivot unscaled).
+__global__ void diag_rsqrt_kernel(int n, const double* __restrict__ A, int64_t lda, double* __restrict__ s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = A[(int64_t)i * lda + i];
+  s[i] = (d > 0.0 && d < INFINITY) ? 1.0 / sqrt(d) : 1.0;
+}
+
+// R = B - A X (ABS: R = |A| w, w = X as a vector or 1 when X is null: the row
+// sums of |A|, weighted), A symmetric from its lower
 // triangle.  Workgroup (w, q): 64 rows of R, 16 columns.  It walks the 64 x 64
 // blocks of row block w of the full symmetric A: blocks left of the diagonal are
 // read from the lower triangle row-wise, blocks right of it as transposed lower
@@ -89,7 +132,7 @@ __global__ __launch_bounds__(256) void symm_kernel(int n, int m, const double* _
     place(j);
     for (int e = tid; e < kNB * MC; e += 256) {
       const int k = e / MC, c = e % MC;
-      Xs[k * SC + c] = ABS ? (k < bj && c == 0 ? 1.0 : 0.0)
+      Xs[k * SC + c] = ABS ? (k < bj && c == 0 ? (X != nullptr ? X[j0 + k] : 1.0) : 0.0)
                            : ((k < bj && c < mc) ? X[(int64_t)(j0 + k) * ldx + c0 + c] : 0.0);
     }
     __syncthreads();
@@ -111,14 +154,16 @@ __global__ __launch_bounds__(256) void symm_kernel(int n, int m, const double* _
   }
 }
 
-// out[c] = max(out[c], max_r |X[r][c]|) for c < cols.  out must start at 0; the
-// maximum of non-negative doubles is the maximum of their bit patterns.
+// out[c] = max(out[c], max_r |X[r][c] w_r|) for c < cols, w_r = rs[r] (or
+// 1 / rs[r] with inv; 1 when rs is null).  out must start at 0; the maximum of
+// non-negative doubles is the maximum of their bit patterns.
 __global__ void colmax_kernel(int rows, int cols, const double* __restrict__ X, int64_t ldx,
-                              unsigned long long* __restrict__ out) {
+                              unsigned long long* __restrict__ out, const double* __restrict__ rs, bool inv) {
   const int c = blockIdx.y;
   double mx = 0.0;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
-    const double v = fabs(X[(int64_t)r * ldx + c]);
+    double v = fabs(X[(int64_t)r * ldx + c]);
+    if (rs != nullptr) v = inv ? v / rs[r] : v * rs[r];
     mx = (v > mx || v != v) ? v : mx;  // a NaN wins, so it cannot look converged
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -135,17 +180,37 @@ dim3 grid_rows(int rows, int cols) {
 }  // namespace
 
 hipError_t launch_convert_d2s(int rows, int cols, const double* X, int64_t ldx, float* Y, int64_t ldy,
-                              bool lower_only, hipStream_t s) {
+                              bool lower_only, hipStream_t s, const double* rs, const double* cs) {
   if (rows <= 0 || cols <= 0) return hipSuccess;
   hipLaunchKernelGGL(convert_d2s_kernel, grid_rows(rows, cols), dim3(256), 0, s, rows, cols, X, ldx, Y, ldy,
-                     lower_only);
+                     lower_only, rs, cs);
   return hipGetLastError();
 }
 
 hipError_t launch_axpy_s2d(int rows, int cols, const float* D, int64_t ldd, double* X, int64_t ldx,
-                           hipStream_t s) {
+                           hipStream_t s, const double* rs) {
   if (rows <= 0 || cols <= 0) return hipSuccess;
-  hipLaunchKernelGGL(axpy_s2d_kernel, grid_rows(rows, cols), dim3(256), 0, s, rows, cols, D, ldd, X, ldx);
+  hipLaunchKernelGGL(axpy_s2d_kernel, grid_rows(rows, cols), dim3(256), 0, s, rows, cols, D, ldd, X, ldx, rs);
+  return hipGetLastError();
+}
+
+hipError_t launch_convert_s2h(int rows, int cols, const float* X, int64_t ldx, _Float16* Y, int64_t ldy,
+                              hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  const int groups = (cols + 7) / 8;
+  const dim3 grid((groups + 255) / 256, rows < 65535 ? rows : 65535, (rows + 65534) / 65535);
+  const bool vec = reinterpret_cast<uintptr_t>(X) % 16 == 0 && reinterpret_cast<uintptr_t>(Y) % 16 == 0 &&
+                   ldx % 4 == 0 && ldy % 8 == 0;
+  if (vec)
+    hipLaunchKernelGGL((convert_s2h_kernel<true>), grid, dim3(256), 0, s, rows, cols, X, ldx, Y, ldy);
+  else
+    hipLaunchKernelGGL((convert_s2h_kernel<false>), grid, dim3(256), 0, s, rows, cols, X, ldx, Y, ldy);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_rsqrt(int n, const double* A, int64_t lda, double* s_out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(diag_rsqrt_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, A, lda, s_out);
   return hipGetLastError();
 }
 
@@ -157,20 +222,22 @@ hipError_t launch_symm_residual(int n, int m, const double* A, int64_t lda, cons
   return hipGetLastError();
 }
 
-hipError_t launch_symm_abs_rowsum(int n, const double* A, int64_t lda, double* out, hipStream_t s) {
+hipError_t launch_symm_abs_rowsum(int n, const double* A, int64_t lda, double* out, hipStream_t s,
+                                  const double* w) {
   if (n <= 0) return hipSuccess;
   const dim3 grid((n + kNB - 1) / kNB, 1);
-  hipLaunchKernelGGL((symm_kernel<true>), grid, dim3(256), 0, s, n, 1, A, lda, nullptr, 0, nullptr, 0, out, 1);
+  hipLaunchKernelGGL((symm_kernel<true>), grid, dim3(256), 0, s, n, 1, A, lda, w, 1, nullptr, 0, out, 1);
   return hipGetLastError();
 }
 
-hipError_t launch_colmax(int rows, int cols, const double* X, int64_t ldx, double* out, hipStream_t s) {
+hipError_t launch_colmax(int rows, int cols, const double* X, int64_t ldx, double* out, hipStream_t s,
+                         const double* rs, bool inv) {
   if (cols <= 0) return hipSuccess;
   hipError_t e = launch_fill_i32(reinterpret_cast<int32_t*>(out), 0, 2 * (int64_t)cols, s);
   if (e != hipSuccess || rows <= 0) return e;
   const int bx = std::min(64, (rows + 255) / 256);
   hipLaunchKernelGGL(colmax_kernel, dim3(bx, cols), dim3(256), 0, s, rows, cols, X, ldx,
-                     reinterpret_cast<unsigned long long*>(out));
+                     reinterpret_cast<unsigned long long*>(out), rs, inv);
   return hipGetLastError();
 }
 

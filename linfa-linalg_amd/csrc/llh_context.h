@@ -25,6 +25,8 @@ struct llh_context {
   Buffer dinv;        // inverses of the 64 x 64 diagonal blocks
   Buffer mixed_l;     // factor storage of the one-shot mixed-precision solve
   Buffer mixed_work;  // its residual / correction / norm buffers
+  Buffer mixed_scale; // its diagonal scaling (f16 factor)
+  Buffer half_panel;  // f16 copy of the panel of the f16 factor's trailing SYRK
   Buffer sync;        // ticket / error / flags of the one-launch solve
   int32_t epoch = 1;
   llh::Lookahead la;  // side stream and events of the factor's lookahead (created on first use)

@@ -35,9 +35,13 @@ struct Lookahead {
 // f64 arithmetic, the mixed-precision form.
 // la (optional): lookahead streams; the whole factor is still complete in
 // stream order on s.
+// half (f32 storage only, optional): scratch of n * kNBO f16 values; each
+// trailing SYRK then rounds its panel to f16 there and multiplies on the f16
+// MFMA with f32 accumulation (the half-precision mixed factor).
 template <typename T>
 hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, T* dinv, hipStream_t s,
-                         Profiler* prof = nullptr, bool diag_f64 = false, const Lookahead* la = nullptr);
+                         Profiler* prof = nullptr, bool diag_f64 = false, const Lookahead* la = nullptr,
+                         _Float16* half = nullptr);
 
 // Enqueues op(A) X = B in place (B row-major n x m); upper_op names the triangle
 // of op(A) and trans != 0 means op(A) = A^T.  dinv, when non-null, holds the

@@ -148,6 +148,12 @@ template <typename T>
 hipError_t launch_syrk_lower(int n, int k, T alpha, const T* A, int64_t lda, T beta, T* C, int64_t ldc,
                              const int32_t* gate, hipStream_t s);
 
+// The same update with f16 operands and f32 accumulation and C: the trailing
+// SYRK of the half-precision mixed factor.  variant selects the tile (0:
+// 128 x 128, 1: 256 x 128).
+hipError_t launch_syrk_lower_h(int n, int k, float alpha, const _Float16* A, int64_t lda, float beta, float* C,
+                               int64_t ldc, const int32_t* gate, hipStream_t s, int variant = 0);
+
 // k_trsm.hip: panel solve X := X * L^-T (X is m x b, L is b x b lower, b <= kNB).
 template <typename T>
 hipError_t launch_trsm_rows(int m, int b, T* X, int64_t ldx, const T* L, int64_t ldl, const int32_t* gate,
@@ -174,14 +180,26 @@ hipError_t launch_trsm_persistent(int n, int m, const T* A, int64_t lda, bool tr
                                   hipStream_t s);
 
 // k_mixed.hip: conversions, symmetric residual and norms of the refinement.
+// Y = diag(rs) X diag(cs) in f32 (null scales are 1).
 hipError_t launch_convert_d2s(int rows, int cols, const double* X, int64_t ldx, float* Y, int64_t ldy,
-                              bool lower_only, hipStream_t s);
+                              bool lower_only, hipStream_t s, const double* rs = nullptr,
+                              const double* cs = nullptr);
+// X += diag(rs) D (null rs is 1).
 hipError_t launch_axpy_s2d(int rows, int cols, const float* D, int64_t ldd, double* X, int64_t ldx,
-                           hipStream_t s);
+                           hipStream_t s, const double* rs = nullptr);
+// Y (f16) = X (f32), round to nearest even.
+hipError_t launch_convert_s2h(int rows, int cols, const float* X, int64_t ldx, _Float16* Y, int64_t ldy,
+                              hipStream_t s);
+// s[i] = 1 / sqrt(A[i][i]) for a positive finite diagonal entry, else 1.
+hipError_t launch_diag_rsqrt(int n, const double* A, int64_t lda, double* s_out, hipStream_t s);
 hipError_t launch_symm_residual(int n, int m, const double* A, int64_t lda, const double* X, int64_t ldx,
                                 const double* B, int64_t ldb, double* R, int64_t ldr, hipStream_t s);
-hipError_t launch_symm_abs_rowsum(int n, const double* A, int64_t lda, double* out, hipStream_t s);
-hipError_t launch_colmax(int rows, int cols, const double* X, int64_t ldx, double* out, hipStream_t s);
+// out = |A| w (w null: 1), A symmetric from its lower triangle.
+hipError_t launch_symm_abs_rowsum(int n, const double* A, int64_t lda, double* out, hipStream_t s,
+                                  const double* w = nullptr);
+// out[c] = max_r |X[r][c]| * rs[r] (inv: / rs[r]; rs null: 1).
+hipError_t launch_colmax(int rows, int cols, const double* X, int64_t ldx, double* out, hipStream_t s,
+                         const double* rs = nullptr, bool inv = false);
 
 // k_util.hip
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t count, hipStream_t s);

@@ -25,6 +25,8 @@ struct llh_factor {
   size_t dinv_bytes = 0;
   void* work = nullptr;       // residual / correction / norms of the solves
   size_t work_bytes = 0;
+  void* scale = nullptr;      // f16 factor: s = diag(A)^-1/2, L factors diag(s) A diag(s)
+  size_t scale_bytes = 0;
   double anrm = 0.0;          // ||A||_inf
 };
 
@@ -38,7 +40,8 @@ namespace {
 
 bool bad_ctx(llh_context* ctx) { return ctx == nullptr || ctx->magic != llh_context::kMagic; }
 
-size_t elem_size(llh_precision p) { return p == LLH_PREC_F32 ? 4 : 8; }
+size_t elem_size(llh_precision p) { return (p == LLH_PREC_F32 || p == LLH_PREC_F16) ? 4 : 8; }
+bool low_precision(llh_precision p) { return p == LLH_PREC_F32 || p == LLH_PREC_F16; }
 
 // Grows an owned or context-borrowed device buffer.
 hipError_t grow_buffer(llh_context* ctx, bool owns, llh_context::Buffer* borrowed, void** ptr, size_t* bytes,
@@ -67,28 +70,47 @@ hipError_t grow_buffer(llh_context* ctx, bool owns, llh_context::Buffer* borrowe
   return hipSuccess;
 }
 
-// Factors f->a into f->l with precision f->prec; on an f32 pivot failure with
-// fallback, refactors in f64.
+// Factors f->a into f->l with precision f->prec; on a low-precision pivot
+// failure with fallback, refactors in f64.  The f16 form factors
+// diag(s) A diag(s) with s = diag(A)^-1/2 (unit diagonal, every entry of L at
+// most 1 in magnitude, so nothing overflows f16), keeping the factor in f32
+// and rounding each trailing SYRK's panel to f16.
 llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error* err) {
   const int n = f->n;
   hipStream_t s = ctx->stream;
-  // ||A||_inf from the lower triangle (f64), once per factor.
   CHECK_HIP(grow_buffer(ctx, f->owns, &ctx->mixed_work, &f->work, &f->work_bytes, (size_t)(n + 2) * 8),
             "work buffer");
   double* rowsum = static_cast<double*>(f->work);
-  CHECK_HIP(launch_symm_abs_rowsum(n, f->a, f->lda, rowsum, s), "row sums");
-  CHECK_HIP(launch_colmax(n, 1, rowsum, 1, rowsum + n, s), "norm");
-  CHECK_HIP(hipMemcpyAsync(&f->anrm, rowsum + n, sizeof(double), hipMemcpyDeviceToHost, s), "norm readback");
   for (int attempt = 0; attempt < 2; ++attempt) {
     const size_t es = elem_size(f->prec);
     CHECK_HIP(grow_buffer(ctx, f->owns, &ctx->mixed_l, &f->l, &f->l_bytes, (size_t)n * n * es), "factor buffer");
     CHECK_HIP(grow_buffer(ctx, f->owns, &ctx->dinv, &f->dinv, &f->dinv_bytes, (size_t)dinv_elems(n) * 8),
               "inverse buffer");
-    if (f->prec == LLH_PREC_F32) {
+    const double* sc = nullptr;
+    if (f->prec == LLH_PREC_F16) {
+      CHECK_HIP(grow_buffer(ctx, f->owns, &ctx->mixed_scale, &f->scale, &f->scale_bytes, (size_t)n * 8),
+                "scale buffer");
+      sc = static_cast<const double*>(f->scale);
+      CHECK_HIP(launch_diag_rsqrt(n, f->a, f->lda, static_cast<double*>(f->scale), s), "scale");
+    }
+    // ||A||_inf (f16: ||diag(s) A diag(s)||_inf, the system it refines) from the
+    // lower triangle, in f64
+    CHECK_HIP(launch_symm_abs_rowsum(n, f->a, f->lda, rowsum, s, sc), "row sums");
+    CHECK_HIP(launch_colmax(n, 1, rowsum, 1, rowsum + n, s, sc), "norm");
+    CHECK_HIP(hipMemcpyAsync(&f->anrm, rowsum + n, sizeof(double), hipMemcpyDeviceToHost, s), "norm readback");
+    if (low_precision(f->prec)) {
       float* l = static_cast<float*>(f->l);
-      CHECK_HIP(launch_convert_d2s(n, n, f->a, f->lda, l, n, true, s), "convert");
+      _Float16* half = nullptr;
+      if (f->prec == LLH_PREC_F16) {
+        void* hp = nullptr;
+        size_t hb = 0;
+        CHECK_HIP(grow_buffer(ctx, false, &ctx->half_panel, &hp, &hb, (size_t)n * kNBO * sizeof(_Float16)),
+                  "f16 panel buffer");
+        half = static_cast<_Float16*>(hp);
+      }
+      CHECK_HIP(launch_convert_d2s(n, n, f->a, f->lda, l, n, true, s, sc, sc), "convert");
       CHECK_HIP(potrf_enqueue<float>(l, n, n, ctx->d_info, false, static_cast<float*>(f->dinv), s,
-                                     ctx->profiler(), true, ctx_lookahead(ctx)),
+                                     ctx->profiler(), true, ctx_lookahead(ctx), half),
                 "potrf f32");
     } else {
       double* l = static_cast<double*>(f->l);
@@ -103,7 +125,7 @@ llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error
     CHECK_HIP(hipStreamSynchronize(s), "synchronize");
     const int32_t fail = ctx->h_info[0];
     if (fail == kNoFail) return ok(err);
-    if (f->prec == LLH_PREC_F32 && (flags & LLH_FACTOR_FALLBACK_F64)) {
+    if (low_precision(f->prec) && (flags & LLH_FACTOR_FALLBACK_F64)) {
       f->prec = LLH_PREC_F64;
       continue;
     }
@@ -143,8 +165,11 @@ llh_status solve_core(llh_context* ctx, llh_factor* f, size_t m_, const double* 
   double hn[128];
   auto residual_check = [&](bool* conv) -> llh_status {
     CHECK_HIP(launch_symm_residual(n, m, f->a, f->lda, x, (int64_t)ldx, b, (int64_t)ldb, r, m, s), "residual");
-    CHECK_HIP(launch_colmax(n, m, r, m, cm, s), "residual norm");
-    CHECK_HIP(launch_colmax(n, m, x, (int64_t)ldx, cm + m, s), "solution norm");
+    // the f16 factor refines diag(s) A diag(s) y = diag(s) b, y = x / s: its
+    // test measures that system's residual and solution
+    const double* rs = f->prec == LLH_PREC_F16 ? static_cast<const double*>(f->scale) : nullptr;
+    CHECK_HIP(launch_colmax(n, m, r, m, cm, s, rs, false), "residual norm");
+    CHECK_HIP(launch_colmax(n, m, x, (int64_t)ldx, cm + m, s, rs, true), "solution norm");
     CHECK_HIP(hipMemcpyAsync(hn, cm, sizeof(double) * 2 * m, hipMemcpyDeviceToHost, s), "norm readback");
     CHECK_HIP(hipStreamSynchronize(s), "synchronize");
     bool c = true;
@@ -171,11 +196,14 @@ llh_status solve_core(llh_context* ctx, llh_factor* f, size_t m_, const double* 
     inf.converged = 1;
   } else {
     if (ldx != (size_t)m) return invalid(err, "ldx must equal m");
+    // The f16 factor is of S A S (S = diag(scale)): A^-1 v = S (S A S)^-1 S v.
+    const double* sc = f->prec == LLH_PREC_F16 ? static_cast<const double*>(f->scale) : nullptr;
     // x = solve_low(b)
-    CHECK_HIP(launch_convert_d2s(n, m, b, (int64_t)ldb, d, m, false, s), "convert rhs");
+    CHECK_HIP(launch_convert_d2s(n, m, b, (int64_t)ldb, d, m, false, s, sc), "convert rhs");
     CHECK_HIP(solve_low<float>(ctx, f, m, d, &sync), "solve f32");
     CHECK_HIP(launch_fill_i32(reinterpret_cast<int32_t*>(x), 0, 2 * (int64_t)n * m, s), "zero x");
-    CHECK_HIP(launch_axpy_s2d(n, m, d, m, x, (int64_t)ldx, s), "update x");
+    CHECK_HIP(launch_axpy_s2d(n, m, d, m, x, (int64_t)ldx, s, sc), "update x");
+    double prev = INFINITY;
     for (int it = 0;; ++it) {
       bool conv = false;
       llh_status st = residual_check(&conv);
@@ -185,9 +213,14 @@ llh_status solve_core(llh_context* ctx, llh_factor* f, size_t m_, const double* 
         break;
       }
       if (it >= max_iter) break;
-      CHECK_HIP(launch_convert_d2s(n, m, r, m, d, m, false, s), "convert residual");
+      // The f16 factor contracts the error by about cond(A) * 2^-11 per step:
+      // a step that does not halve the residual means A is too ill-conditioned
+      // for it, and the caller is better served by the fallback at once.
+      if (f->prec == LLH_PREC_F16 && !(inf.residual <= 0.5 * prev)) break;
+      prev = inf.residual;
+      CHECK_HIP(launch_convert_d2s(n, m, r, m, d, m, false, s, sc), "convert residual");
       CHECK_HIP(solve_low<float>(ctx, f, m, d, &sync), "solve f32");
-      CHECK_HIP(launch_axpy_s2d(n, m, d, m, x, (int64_t)ldx, s), "update x");
+      CHECK_HIP(launch_axpy_s2d(n, m, d, m, x, (int64_t)ldx, s, sc), "update x");
       inf.iterations = it + 1;
     }
   }
@@ -218,7 +251,7 @@ llh_status llh_factor_create(llh_context* ctx, llh_precision prec, size_t n, con
   *out = nullptr;
   llh_status st = check_factor_args(ctx, n, a, lda, err);
   if (st != LLH_OK) return st;
-  if (prec != LLH_PREC_F64 && prec != LLH_PREC_F32) return invalid(err, "unknown precision");
+  if (prec != LLH_PREC_F64 && !low_precision(prec)) return invalid(err, "unknown precision");
   auto* f = new llh_factor();
   f->prec = prec;
   f->n = (int)n;
@@ -242,7 +275,7 @@ llh_status llh_factor_solve(llh_context* ctx, llh_factor* f, size_t m, const dou
   if (b == nullptr || x == nullptr || ldb < m || ldx < m) return invalid(err, "bad right-hand side arguments");
   llh_status st = solve_core(ctx, f, m, b, ldb, x, ldx, opts, info, err);
   if (st == LLH_ERR_NO_CONVERGENCE && opts && (opts->flags & LLH_FACTOR_FALLBACK_F64) &&
-      f->prec == LLH_PREC_F32) {
+      low_precision(f->prec)) {
     f->prec = LLH_PREC_F64;
     st = factor_core(ctx, f, 0, err);
     if (st != LLH_OK) return st;
@@ -258,7 +291,7 @@ llh_precision llh_factor_precision(const llh_factor* f) {
 
 size_t llh_factor_device_bytes(const llh_factor* f) {
   if (!f || f->magic != llh_factor::kMagic) return 0;
-  return f->l_bytes + f->dinv_bytes + f->work_bytes;
+  return f->l_bytes + f->dinv_bytes + f->work_bytes + f->scale_bytes;
 }
 
 void llh_factor_destroy(llh_factor* f) {
@@ -267,6 +300,7 @@ void llh_factor_destroy(llh_factor* f) {
     if (f->l) (void)hipFree(f->l);
     if (f->dinv) (void)hipFree(f->dinv);
     if (f->work) (void)hipFree(f->work);
+    if (f->scale) (void)hipFree(f->scale);
   }
   f->magic = 0;
   delete f;
@@ -277,7 +311,7 @@ llh_status llh_dev_solve_mixed(llh_context* ctx, llh_precision prec, size_t n, c
                                const llh_refine_opts* opts, llh_refine_info* info, llh_error* err) {
   llh_status st = check_factor_args(ctx, n, a, lda, err);
   if (st != LLH_OK) return st;
-  if (prec != LLH_PREC_F64 && prec != LLH_PREC_F32) return invalid(err, "unknown precision");
+  if (prec != LLH_PREC_F64 && !low_precision(prec)) return invalid(err, "unknown precision");
   llh_factor f;
   f.prec = prec;
   f.n = (int)n;
@@ -286,7 +320,7 @@ llh_status llh_dev_solve_mixed(llh_context* ctx, llh_precision prec, size_t n, c
   f.owns = false;
   const int32_t flags = opts ? opts->flags : 0;
   st = factor_core(ctx, &f, flags, err);
-  const bool fell = prec == LLH_PREC_F32 && f.prec == LLH_PREC_F64;
+  const bool fell = low_precision(prec) && f.prec == LLH_PREC_F64;
   if (st != LLH_OK) return st;
   if (m > 0) {
     st = llh_factor_solve(ctx, &f, m, b, ldb, x, ldx, opts, info, err);

@@ -71,7 +71,7 @@ class ServerStats(ctypes.Structure):
                                                "steps", "bytes_in_use", "bytes_peak", "resident_factors")]
 
 
-PREC = {"f64": 0, "f32": 1, "f32ir": 2}
+PREC = {"f64": 0, "f32": 1, "f32ir": 2, "f16": 3}
 FALLBACK_F64 = 1
 
 KERNEL_CLASSES = {"trailing_syrk": 0, "panel_update": 1, "diag_factor": 2, "panel_trsm": 3,

@@ -133,8 +133,10 @@ def _opts(max_iter, fallback):
 
 def solve_mixed(a, b, x, prec="f32", max_iter=30, fallback=True):
     """x = a^-1 b for an f64 SPD a (lower triangle read), factored in `prec`
-    with f64 iterative refinement (llh_dev_solve_mixed).  Returns the
-    refinement report."""
+    with f64 iterative refinement (llh_dev_solve_mixed).  prec "f32": f32
+    trailing updates; "f16": f16 trailing updates (f32 accumulation) of the
+    diagonally scaled a, for well-conditioned a; "f64": no refinement.  Returns
+    the refinement report."""
     for t in (a, b, x):
         _check(t)
     n, m = b.shape

@@ -92,3 +92,104 @@ def test_not_positive_definite():
     with pytest.raises(llh.NotPositiveDefinite) as ei:
         dev.solve_mixed(tt(a), tt(np.ones((n, 1))), x, prec="f32", fallback=True)
     assert ei.value.column == 150
+
+
+# --- f16 trailing updates (LLH_PREC_F16) --------------------------------------
+
+@pytest.mark.parametrize("n", [1600, 2100, 3000])
+@pytest.mark.parametrize("lam", [1.0, 1e-1])
+def test_f16_ir_matches_f64_oracle(n, lam):
+    """Well-conditioned systems (cond ~5 and ~40): the f16-updated factor
+    refines to f64 accuracy, with more steps than the f32 one.  (The first
+    512-column panel is updated in f32 by the look-ahead, so f16 products
+    appear from the second outer step on: n > 1024 is needed to meet them.)"""
+    from linfa_linalg_hip import device as dev
+    import torch
+    a = spd_normal(n, lam, n + 7)
+    b = np.random.default_rng(n + 8).standard_normal((n, 16))
+    x = torch.zeros((n, 16), dtype=torch.float64, device="cuda")
+    info = dev.solve_mixed(tt(a), tt(b), x, prec="f16", fallback=False)
+    assert info["converged"] and not info["fell_back"] and info["precision_used"] == "f16", info
+    assert info["iterations"] >= 2
+    x_ref = orc.solvec(np.array(a), b)
+    err = np.linalg.norm(x.cpu().numpy() - x_ref) / np.linalg.norm(x_ref)
+    assert err <= 1e-12, (err, info)
+    assert info["residual"] <= 10 * np.sqrt(n) * np.finfo(np.float64).eps
+
+
+def test_f16_diagonal_scaling():
+    """A = D A0 D with D from 1e-4 to 1e4: unscaled, L's entries would span
+    ~1e-6 .. 1e4, the small ones below f16's normal range; the factor of
+    diag(s) A diag(s) has a unit diagonal and |L| <= 1.  A is ill-conditioned
+    only through D, so x is compared with a known solution in both the plain
+    and the D-scaled norm, against the oracle's own errors."""
+    from linfa_linalg_hip import device as dev
+    import torch
+    n = 2100
+    rng = np.random.default_rng(22)
+    d = np.logspace(-4, 4, n)
+    a0 = spd_normal(n, 1.0, 21)
+    a = (d[:, None] * a0) * d[None, :]
+    x_true = rng.standard_normal((n, 4)) / d[:, None]
+    b = a @ x_true
+    x = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    info = dev.solve_mixed(tt(a), tt(b), x, prec="f16", fallback=False)
+    assert info["converged"] and info["precision_used"] == "f16", info
+    assert info["residual"] <= 10 * np.sqrt(n) * np.finfo(np.float64).eps
+    xs = x.cpu().numpy()
+    x_ref = orc.solvec(np.array(a), b)
+    for w in (np.ones(n), d):  # plain and D-scaled errors
+        e_gpu = np.linalg.norm(w[:, None] * (xs - x_true)) / np.linalg.norm(w[:, None] * x_true)
+        e_orc = np.linalg.norm(w[:, None] * (x_ref - x_true)) / np.linalg.norm(w[:, None] * x_true)
+        assert e_gpu <= 10 * e_orc + 1e-13, (e_gpu, e_orc, info)
+
+
+def test_f16_factor_reuse():
+    from linfa_linalg_hip import device as dev
+    import torch
+    n = 1300
+    a = spd_normal(n, 1.0, 31)
+    ta = tt(a)
+    f = dev.Factor(ta, prec="f16", fallback=False)
+    assert f.precision == "f16"
+    for seed in (1, 2):
+        b = np.random.default_rng(seed).standard_normal((n, 3))
+        x = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+        info = f.solve(tt(b), x)
+        assert info["converged"], info
+        x_ref = orc.solvec(np.array(a), b)
+        assert np.linalg.norm(x.cpu().numpy() - x_ref) / np.linalg.norm(x_ref) <= 1e-12
+    f.close()
+
+
+def test_f16_ill_conditioned_falls_back():
+    """Eigenvalues down to 1e-12: far beyond what the f16-updated factor
+    refines (it breaks down, or the stall test ends the refinement); without
+    fallback that is an error, with it the f64 factor produces the result."""
+    from linfa_linalg_hip import device as dev
+    import torch
+    n = 1600
+    rng = np.random.default_rng(41)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    a = (q * np.logspace(0, -12, n)) @ q.T
+    a = (a + a.T) / 2
+    b = rng.standard_normal((n, 2))
+    x = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+    with pytest.raises(llh.LinalgError):
+        dev.solve_mixed(tt(a), tt(b), x, prec="f16", fallback=False, max_iter=10)
+    x = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+    info = dev.solve_mixed(tt(a), tt(b), x, prec="f16", fallback=True)
+    assert info["fell_back"] and info["precision_used"] == "f64" and info["converged"], info
+    assert info["residual"] <= 10 * np.sqrt(n) * np.finfo(np.float64).eps
+
+
+def test_f16_not_positive_definite():
+    from linfa_linalg_hip import device as dev
+    import torch
+    n = 1700
+    a = spd_normal(n, 1.0, 51)
+    a[1600, 1600] = -5.0
+    x = torch.zeros((n, 1), dtype=torch.float64, device="cuda")
+    with pytest.raises(llh.NotPositiveDefinite) as ei:
+        dev.solve_mixed(tt(a), tt(np.ones((n, 1))), x, prec="f16", fallback=False)
+    assert ei.value.column == 1600
