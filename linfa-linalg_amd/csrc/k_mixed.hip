@@ -2,6 +2,8 @@
 // r = b - A x of a symmetric A held in its lower triangle (the part the
 // reference reads, cholesky.rs:58-66), and the column norms of the refinement
 // test.
+#include <cstdint>
+
 #include "llh_common.h"
 #include "llh_kernels.h"
 
@@ -68,90 +70,183 @@ __global__ void diag_rsqrt_kernel(int n, const double* __restrict__ A, int64_t l
   s[i] = (d > 0.0 && d < INFINITY) ? 1.0 / sqrt(d) : 1.0;
 }
 
-// R = B - A X (ABS: R = |A| w, w = X as a vector or 1 when X is null: the row
-// sums of |A|, weighted), A symmetric from its lower
-// triangle.  Workgroup (w, q): 64 rows of R, 16 columns.  It walks the 64 x 64
-// blocks of row block w of the full symmetric A: blocks left of the diagonal are
-// read from the lower triangle row-wise, blocks right of it as transposed lower
-// blocks, the diagonal block from both halves.  The next block is prefetched into
-// registers while the current one is applied.
-template <bool ABS>
-__global__ __launch_bounds__(256) void symm_kernel(int n, int m, const double* __restrict__ A, int64_t lda,
-                                                   const double* __restrict__ X, int64_t ldx,
-                                                   const double* __restrict__ B, int64_t ldb,
-                                                   double* __restrict__ R, int64_t ldr) {
-  constexpr int S = kNB + 1, MC = 16, SC = MC + 1;
-  __shared__ double As[kNB * S];
-  __shared__ double Xs[kNB * SC];
-  const int tid = threadIdx.x;
-  const int w = blockIdx.x, q = blockIdx.y;
+// R = B - A X (ABS: R = |A| w, w = X as a vector or 1 when X is null: the
+// weighted row sums of |A|), A symmetric from its lower triangle.
+//
+// Workgroup (i, q, sp): 64 rows of R (row block i), 16 columns (chunk q), and
+// the blocks j of split sp of row block i of the full symmetric A.  Lane r of
+// wave v holds row r's 16 values of block j at k = 16v .. 16v + 15 in
+// registers, loaded two blocks ahead: blocks left of the diagonal row-wise from
+// the lower triangle, blocks right of it as transposed lower blocks (lane r
+// reads column i0 + r, so a wave's load is one contiguous row segment), the
+// diagonal block from both halves.  X_j goes through a double-buffered LDS slab,
+// one barrier per block; the four wave partials are reduced once at the end.
+// With one split the workgroup writes R; with several it writes its partial sum
+// to P[sp] and symm_reduce_kernel adds the splits in order, so the result does
+// not depend on scheduling either way.
+constexpr int kSymmMC = 16;
+
+template <bool ABS, bool VEC>
+__global__ __launch_bounds__(256) void symm_walk_kernel(int n, int m, const double* __restrict__ A, int64_t lda,
+                                                        const double* __restrict__ X, int64_t ldx,
+                                                        const double* __restrict__ B, int64_t ldb,
+                                                        double* __restrict__ R, int64_t ldr,
+                                                        double* __restrict__ P, int nsplit, int mp) {
+  constexpr int MC = kSymmMC, KW = kNB / 4, SL = kNB * MC;
+  __shared__ __attribute__((aligned(16))) double Ys[2][SL];
+  __shared__ double Part[3][SL];
+  const int i = blockIdx.x, q = blockIdx.y, sp = blockIdx.z;
   const int nb = (n + kNB - 1) / kNB;
-  const int w0 = w * kNB, bw = min(kNB, n - w0);
+  const int per = (nb + nsplit - 1) / nsplit;
+  const int jb = sp * per, je = min(nb, jb + per);
+  const int i0 = i * kNB, bi = min(kNB, n - i0);
   const int c0 = q * MC, mc = ABS ? 1 : min(MC, m - c0);
-  const int r = tid >> 2, cc = (tid & 3) * 4;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  double pre[16];
-  // Element e = tid + 256 p of block j: (i, jj) = (e / 64, e % 64) of the stored
-  // 64 x 64 tile that is read coalesced; it lands at As[rr][kk].
-  auto fetch = [&](int j) {
+  const int tid = threadIdx.x, r = tid & 63, wv = tid >> 6, k0 = KW * wv;
+  const int rr = min(r, bi - 1);  // clamped row: rows past the matrix compute and are not written
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  auto fetch = [&](int j, double* out) {
     const int j0 = j * kNB, bj = min(kNB, n - j0);
+    if (j < i) {  // A[i0 + r][j0 + k]
+      const double* row = A + (int64_t)(i0 + rr) * lda + j0 + k0;
+      if (VEC && k0 + KW <= bj) {
 #pragma unroll
-    for (int p = 0; p < 16; ++p) {
-      const int e = tid + 256 * p, i = e / kNB, jj = e % kNB;
-      double v = 0.0;
-      if (j < w) {  // A[w0+i][j0+jj], lower
-        if (i < bw && jj < bj) v = A[(int64_t)(w0 + i) * lda + j0 + jj];
-      } else if (j > w) {  // A[j0+i][w0+jj] = As[jj][i]
-        if (i < bj && jj < bw) v = A[(int64_t)(j0 + i) * lda + w0 + jj];
-      } else {  // diagonal block: lower part read once, mirrored on store
-        if (i < bw && jj <= i) v = A[(int64_t)(w0 + i) * lda + w0 + jj];
-      }
-      pre[p] = v;
-    }
-  };
-  auto place = [&](int j) {
-#pragma unroll
-    for (int p = 0; p < 16; ++p) {
-      const int e = tid + 256 * p, i = e / kNB, jj = e % kNB;
-      const double v = ABS ? fabs(pre[p]) : pre[p];
-      if (j < w) {
-        As[i * S + jj] = v;
-      } else if (j > w) {
-        As[jj * S + i] = v;
-      } else {
-        if (jj <= i) {
-          As[i * S + jj] = v;
-          As[jj * S + i] = v;
+        for (int v = 0; v < KW / 2; ++v) {
+          const v2d x = *reinterpret_cast<const v2d*>(row + 2 * v);
+          out[2 * v] = x[0];
+          out[2 * v + 1] = x[1];
         }
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < KW; ++kk) out[kk] = (k0 + kk < bj) ? row[kk] : 0.0;
+      }
+    } else if (j > i) {  // A[j0 + k][i0 + r]
+      const double* col = A + (int64_t)(j0 + k0) * lda + i0 + rr;
+#pragma unroll
+      for (int kk = 0; kk < KW; ++kk) out[kk] = (k0 + kk < bj) ? col[(int64_t)kk * lda] : 0.0;
+    } else {  // diagonal block, from its lower half
+#pragma unroll
+      for (int kk = 0; kk < KW; ++kk) {
+        const int k = k0 + kk;
+        const int64_t o = k <= rr ? (int64_t)(i0 + rr) * lda + i0 + k : (int64_t)(i0 + min(k, bi - 1)) * lda + i0 + rr;
+        const double v = A[o];
+        out[kk] = k < bi ? v : 0.0;
       }
     }
+    if (ABS) {
+#pragma unroll
+      for (int kk = 0; kk < KW; ++kk) out[kk] = fabs(out[kk]);
+    }
   };
-  fetch(0);
-  for (int j = 0; j < nb; ++j) {
+  // X_j rows [0, 64) x chunk columns: 4 values per thread (row e / 16, column e % 16)
+  auto loady = [&](int j, double* out) {
     const int j0 = j * kNB, bj = min(kNB, n - j0);
-    place(j);
-    for (int e = tid; e < kNB * MC; e += 256) {
-      const int k = e / MC, c = e % MC;
-      Xs[k * SC + c] = ABS ? (k < bj && c == 0 ? (X != nullptr ? X[j0 + k] : 1.0) : 0.0)
-                           : ((k < bj && c < mc) ? X[(int64_t)(j0 + k) * ldx + c0 + c] : 0.0);
-    }
-    __syncthreads();
-    if (j + 1 < nb) fetch(j + 1);
-#pragma unroll 8
-    for (int k = 0; k < kNB; ++k) {
-      const double a = As[r * S + k];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) acc[e] += a * Xs[k * SC + cc + e];
+    for (int g = 0; g < 4; ++g) {
+      const int e = tid + 256 * g, k = e / MC, c = e % MC;
+      if (ABS)
+        out[g] = (k < bj && c == 0) ? (X != nullptr ? X[j0 + k] : 1.0) : 0.0;
+      else
+        out[g] = (k < bj && c < mc) ? X[(int64_t)(j0 + k) * ldx + c0 + c] : 0.0;
     }
-    __syncthreads();
-  }
+  };
+
+  double acc[MC];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    if (r < bw && cc + e < mc) {
-      const int64_t row = w0 + r;
-      R[row * ldr + c0 + cc + e] = ABS ? acc[e] : B[row * ldb + c0 + cc + e] - acc[e];
+  for (int c = 0; c < MC; ++c) acc[c] = 0.0;
+  double a0[KW], a1[KW], a2[KW], y0[4], y1[4];
+  if (jb < je) {
+    fetch(jb, a0);
+    loady(jb, y0);
+  }
+  if (jb + 1 < je) {
+    fetch(jb + 1, a1);
+    loady(jb + 1, y1);
+  }
+  for (int j = jb; j < je; ++j) {
+    double* ys = Ys[(j - jb) & 1];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) ys[tid + 256 * e] = y0[e];
+    __syncthreads();
+    if (j + 2 < je) fetch(j + 2, a2);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) y0[e] = y1[e];
+    if (j + 2 < je) loady(j + 2, y1);
+    const double* yk = ys + k0 * MC;
+#pragma unroll
+    for (int kk = 0; kk < KW; ++kk) {
+      const double a = a0[kk];
+#pragma unroll
+      for (int c = 0; c < MC; ++c) acc[c] += a * yk[kk * MC + c];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KW; ++kk) {
+      a0[kk] = a1[kk];
+      a1[kk] = a2[kk];
     }
   }
+  __syncthreads();
+  if (wv > 0) {
+#pragma unroll
+    for (int c = 0; c < MC; ++c) Part[wv - 1][r * MC + c] = acc[c];
+  }
+  __syncthreads();
+  if (wv == 0 && r < bi) {
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+      if (c < mc) {
+        const double sum = acc[c] + Part[0][r * MC + c] + Part[1][r * MC + c] + Part[2][r * MC + c];
+        const int64_t row = i0 + r;
+        if (nsplit == 1)
+          R[row * ldr + c0 + c] = ABS ? sum : B[row * ldb + c0 + c] - sum;
+        else
+          P[((int64_t)sp * n + row) * mp + c0 + c] = sum;
+      }
+    }
+  }
+}
+
+// R[row][c] = B[row][c] - sum_sp P[sp][row][c] (ABS: the sum), splits in order.
+template <bool ABS>
+__global__ void symm_reduce_kernel(int n, int m, int nsplit, int mp, const double* __restrict__ P,
+                                   const double* __restrict__ B, int64_t ldb, double* __restrict__ R,
+                                   int64_t ldr) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * m) return;
+  const int row = (int)(e / m), c = (int)(e % m);
+  double sum = 0.0;
+  for (int sp = 0; sp < nsplit; ++sp) sum += P[((int64_t)sp * n + row) * mp + c];
+  R[(int64_t)row * ldr + c] = ABS ? sum : B[(int64_t)row * ldb + c] - sum;
+}
+
+// Splits of the symmetric walk: about 2048 workgroups, each with at least 4
+// blocks, within the scratch given.
+int symm_splits(int n, int m, int64_t scratch_elems) {
+  const int nb = (n + kNB - 1) / kNB, nq = (m + kSymmMC - 1) / kSymmMC, mp = nq * kSymmMC;
+  int sp = std::max(1, std::min(2048 / std::max(1, nb * nq), nb / 4));
+  while (sp > 1 && (int64_t)sp * n * mp > scratch_elems) --sp;
+  return sp;
+}
+
+template <bool ABS>
+hipError_t launch_symm(int n, int m, const double* A, int64_t lda, const double* X, int64_t ldx, const double* B,
+                       int64_t ldb, double* R, int64_t ldr, double* scratch, int64_t scratch_elems,
+                       hipStream_t s) {
+  const int nb = (n + kNB - 1) / kNB, nq = (m + kSymmMC - 1) / kSymmMC, mp = nq * kSymmMC;
+  const int sp = symm_splits(n, m, scratch != nullptr ? scratch_elems : 0);
+  const bool vec = reinterpret_cast<uintptr_t>(A) % 16 == 0 && lda % 2 == 0;
+  const dim3 grid(nb, nq, sp);
+  if (vec)
+    hipLaunchKernelGGL((symm_walk_kernel<ABS, true>), grid, dim3(256), 0, s, n, m, A, lda, X, ldx, B, ldb, R, ldr,
+                       scratch, sp, mp);
+  else
+    hipLaunchKernelGGL((symm_walk_kernel<ABS, false>), grid, dim3(256), 0, s, n, m, A, lda, X, ldx, B, ldb, R, ldr,
+                       scratch, sp, mp);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || sp == 1) return e;
+  const int64_t total = (int64_t)n * m;
+  hipLaunchKernelGGL((symm_reduce_kernel<ABS>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, n, m, sp, mp,
+                     scratch, B, ldb, R, ldr);
+  return hipGetLastError();
 }
 
 // out[c] = max(out[c], max_r |X[r][c] w_r|) for c < cols, w_r = rs[r] (or
@@ -214,20 +309,22 @@ hipError_t launch_diag_rsqrt(int n, const double* A, int64_t lda, double* s_out,
   return hipGetLastError();
 }
 
+int64_t symm_scratch_elems(int n, int m) {
+  const int nq = (m + kSymmMC - 1) / kSymmMC;
+  return (int64_t)symm_splits(n, m, INT64_MAX) * n * nq * kSymmMC;
+}
+
 hipError_t launch_symm_residual(int n, int m, const double* A, int64_t lda, const double* X, int64_t ldx,
-                                const double* B, int64_t ldb, double* R, int64_t ldr, hipStream_t s) {
+                                const double* B, int64_t ldb, double* R, int64_t ldr, hipStream_t s,
+                                double* scratch, int64_t scratch_elems) {
   if (n <= 0 || m <= 0) return hipSuccess;
-  const dim3 grid((n + kNB - 1) / kNB, (m + 15) / 16);
-  hipLaunchKernelGGL((symm_kernel<false>), grid, dim3(256), 0, s, n, m, A, lda, X, ldx, B, ldb, R, ldr);
-  return hipGetLastError();
+  return launch_symm<false>(n, m, A, lda, X, ldx, B, ldb, R, ldr, scratch, scratch_elems, s);
 }
 
 hipError_t launch_symm_abs_rowsum(int n, const double* A, int64_t lda, double* out, hipStream_t s,
-                                  const double* w) {
+                                  const double* w, double* scratch, int64_t scratch_elems) {
   if (n <= 0) return hipSuccess;
-  const dim3 grid((n + kNB - 1) / kNB, 1);
-  hipLaunchKernelGGL((symm_kernel<true>), grid, dim3(256), 0, s, n, 1, A, lda, w, 1, nullptr, 0, out, 1);
-  return hipGetLastError();
+  return launch_symm<true>(n, 1, A, lda, w, 1, nullptr, 0, out, 1, scratch, scratch_elems, s);
 }
 
 hipError_t launch_colmax(int rows, int cols, const double* X, int64_t ldx, double* out, hipStream_t s,

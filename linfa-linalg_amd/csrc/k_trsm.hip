@@ -140,25 +140,40 @@ __global__ __launch_bounds__(256) void trtri64_kernel(int n, const TS* __restric
 // values in registers, loaded from global memory one block ahead; X_j goes
 // through a double-buffered LDS slab with one barrier per block; and flags are
 // checked in windows: one poll and one acquire fence cover every block whose
-// flag is already raised.  The wave partial sums are reduced once, at the end.
+// flag is already raised.  The wave partial sums are reduced once.
+//
+// Only the last dependencies lie on the chain from flag to flag, so the
+// workgroup does everything else before it waits for them.  With l1 and l2 the
+// blocks solved just before w (l1 last), it forms at the start
+// G_i = inv(op(A)_ww) op(A)_w,li (in LDS) and B_w; after the other blocks,
+// Z = inv(op(A)_ww) (B_w - sum_{j != l1, l2} op(A)_wj X_j); once X_l2 is
+// published Z -= G_2 X_l2, and once X_l1 is, X_w = Z - G_1 X_l1.  So a flag is
+// followed by one 64 x 64 product and one reduction before the next is raised,
+// and the rest of the block's work overlaps the two steps before it.
 // The wait is bounded: past spin_limit polls (or once another workgroup has
 // given up) the workgroup records an error and returns without raising its flag.
 constexpr int kSolveChunk = kSolveChunkHost;
 constexpr int kFlagWindow = 64;  // flags checked ahead per wait
 
+// f64: the three 64 x 64 tiles and the work area take 139 KB of LDS, one
+// workgroup per CU; f32: two.
 template <typename T, bool VEC>
-__global__ __launch_bounds__(256, 2) void trsm_persistent_kernel(int n, int m, const T* __restrict__ A,
-                                                                 int64_t lda, bool trans, bool lower_op,
-                                                                 const T* __restrict__ Dinv, bool dinv_t,
-                                                                 T* B, int64_t ldb, int32_t* ticket,
-                                                                 int32_t* flags, int32_t epoch, int32_t* err,
-                                                                 int64_t spin_limit) {
-  constexpr int MC = kSolveChunk, KW = kNB / 4, S = kNB + 1;
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void trsm_persistent_kernel(
+    int n, int m, const T* __restrict__ A, int64_t lda, bool trans, bool lower_op, const T* __restrict__ Dinv,
+    bool dinv_t, T* B, int64_t ldb, int32_t* ticket, int32_t* flags, int32_t epoch, int32_t* err,
+    int64_t spin_limit) {
+  constexpr int MC = kSolveChunk, KW = kNB / 4, S = kNB + 1, SL = kNB * MC;
   static_assert(MC == 16 && KW == 16, "one 16-column chunk, 16 tile columns per wave");
-  __shared__ __attribute__((aligned(16))) T Ys[2][kNB * MC];  // X_j slabs (row k, column c)
-  __shared__ __attribute__((aligned(16))) T Part[3][kNB * MC];  // partial sums of waves 1..3
-  __shared__ T Dl[kNB * S];                                     // inv(op(A)_ww)
+  static_assert(5 * SL >= kNB * S, "a staged tile fits the work area");
+  // Work area, by phase: a staged tile of op(A) (64 x S) for G; the
+  // double-buffered X_j slabs [0, 2 SL) of the walk; the 4 wave partials
+  // [0, 4 SL) and one slab [4 SL, 5 SL) of the reductions.
+  __shared__ __attribute__((aligned(16))) T Wk[5 * SL];
+  __shared__ T Dl[kNB * S];              // inv(op(A)_ww)
+  __shared__ T Gs[2][kNB * S];           // G_1, G_2
   __shared__ int s_ticket, s_ready, s_abort;
+  T* const Part = Wk;
+  T* const Slab = Wk + 4 * SL;
   const int tid = threadIdx.x;
   const int nb = (n + kNB - 1) / kNB, nq = (m + MC - 1) / MC;
   if (tid == 0) {
@@ -183,6 +198,13 @@ __global__ __launch_bounds__(256, 2) void trsm_persistent_kernel(int n, int m, c
       const int e = tid + 256 * g, i = e / kNB, jj = e % kNB;
       Dl[(dinv_t ? jj : i) * S + (dinv_t ? i : jj)] = D[e];
     }
+  }
+  // B_w[r][4 wv .. 4 wv + 3], the entries this thread finishes
+  T bwr[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int c = 4 * wv + e;
+    bwr[e] = (r < bw && c < mc) ? B[(int64_t)(w0 + r) * ldb + c0 + c] : T(0);
   }
 
   // this lane's 16 values of op(A)[w0 + r, j0 + 16 wv + kk] for block j
@@ -218,89 +240,150 @@ __global__ __launch_bounds__(256, 2) void trsm_persistent_kernel(int n, int m, c
       out[g] = (k < bj && c < mc) ? B[(int64_t)(j0 + k) * ldb + c0 + c] : T(0);
     }
   };
+  // wait until block st's flag is raised, then take every later flag already
+  // raised (up to the window); false when the wait gave up
+  int ready = 0;
+  auto wait_for = [&](int st) -> bool {
+    if (tid == 0) {
+      int64_t it = 0;
+      const int32_t* f = &flags[jof(st) * nq + q];
+      while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++it > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          atomicExch(err, 1);
+          s_abort = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      int t2 = st + 1;
+      while (t2 < p && t2 < st + kFlagWindow &&
+             __hip_atomic_load(&flags[jof(t2) * nq + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+        ++t2;
+      s_ready = t2;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    if (s_abort) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    ready = s_ready;
+    return true;
+  };
 
+  // Gs[i][r][16 wv + kk] = sum_k inv(op(A)_ww)[r][k] op(A)_w,l[k][16 wv + kk], l at position st
+  auto make_g = [&](int st, T* gs) {
+    T t[KW];
+    fetch(st, t);
+#pragma unroll
+    for (int kk = 0; kk < KW; ++kk) Wk[r * S + KW * wv + kk] = t[kk];
+    __syncthreads();
+    T g[KW];
+#pragma unroll
+    for (int kk = 0; kk < KW; ++kk) g[kk] = T(0);
+#pragma unroll 4
+    for (int k = 0; k < kNB; ++k) {
+      const T d = Dl[r * S + k];
+#pragma unroll
+      for (int kk = 0; kk < KW; ++kk) g[kk] += d * Wk[k * S + KW * wv + kk];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KW; ++kk) gs[r * S + KW * wv + kk] = g[kk];
+    __syncthreads();
+  };
+  const int l1 = p - 1, l2 = p - 2;  // positions of the last two dependencies
+  if (p > 0) make_g(l1, Gs[0]);
+  if (p > 1) make_g(l2, Gs[1]);
+
+  // the walk over the blocks before l2
+  const int nwalk = max(p - 2, 0);
   T acc[MC];
 #pragma unroll
   for (int c = 0; c < MC; ++c) acc[c] = T(0);
-  T a_cur[KW], a_nxt[KW], y_cur[4], y_nxt[4];
-  int ready = 0;
-  bool y_pre = false;
-  if (p > 0) fetch(0, a_cur);
-  for (int st = 0; st < p; ++st) {
-    if (st >= ready) {
-      // wait for block st's flag, then take every later flag already raised
-      if (tid == 0) {
-        int64_t it = 0;
-        const int32_t* f = &flags[jof(st) * nq + q];
-        while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-          if (++it > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-            atomicExch(err, 1);
-            s_abort = 1;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
-        int t2 = st + 1;
-        while (t2 < p && t2 < st + kFlagWindow &&
-               __hip_atomic_load(&flags[jof(t2) * nq + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
-          ++t2;
-        s_ready = t2;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  {
+    T a_cur[KW], a_nxt[KW], y_cur[4], y_nxt[4];
+    bool y_pre = false;
+    if (nwalk > 0) fetch(0, a_cur);
+    for (int st = 0; st < nwalk; ++st) {
+      if (st >= ready) {
+        if (!wait_for(st)) return;
+        y_pre = false;
       }
+      if (y_pre) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) y_cur[e] = y_nxt[e];
+      } else {
+        loady(st, y_cur);
+      }
+      T* ys = Wk + (st & 1) * SL;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) ys[tid + 256 * e] = y_cur[e];
       __syncthreads();
-      if (s_abort) return;
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      ready = s_ready;
-      y_pre = false;
+      if (st + 1 < nwalk) fetch(st + 1, a_nxt);
+      y_pre = st + 1 < nwalk && st + 1 < ready;
+      if (y_pre) loady(st + 1, y_nxt);
+      const T* yk = ys + KW * wv * MC;
+#pragma unroll
+      for (int kk = 0; kk < KW; ++kk) {
+        const T a = a_cur[kk];
+#pragma unroll
+        for (int c = 0; c < MC; ++c) acc[c] += a * yk[kk * MC + c];
+      }
+#pragma unroll
+      for (int kk = 0; kk < KW; ++kk) a_cur[kk] = a_nxt[kk];
     }
-    if (y_pre) {
-#pragma unroll
-      for (int g = 0; g < 4; ++g) y_cur[g] = y_nxt[g];
-    } else {
-      loady(st, y_cur);
-    }
-    T* ys = Ys[st & 1];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) ys[tid + 256 * g] = y_cur[g];
-    __syncthreads();
-    if (st + 1 < p) fetch(st + 1, a_nxt);
-    y_pre = st + 1 < ready;
-    if (y_pre) loady(st + 1, y_nxt);
-    const T* yk = ys + KW * wv * MC;
-#pragma unroll
-    for (int kk = 0; kk < KW; ++kk) {
-      const T a = a_cur[kk];
-#pragma unroll
-      for (int c = 0; c < MC; ++c) acc[c] += a * yk[kk * MC + c];
-    }
-#pragma unroll
-    for (int kk = 0; kk < KW; ++kk) a_cur[kk] = a_nxt[kk];
   }
 
-  // reduce the wave partials: Ys[0] = B_w - sum (rows r < bw, columns c < mc)
+  // Z = inv(op(A)_ww) (B_w - sum): reduce the wave partials into Slab, then
+  // lane r, columns 4 wv .. 4 wv + 3
   __syncthreads();
-  if (wv > 0) {
 #pragma unroll
-    for (int c = 0; c < MC; ++c) Part[wv - 1][r * MC + c] = acc[c];
+  for (int c = 0; c < MC; ++c) Part[wv * SL + r * MC + c] = acc[c];
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int c = 4 * wv + e, o = r * MC + c;
+    const T sum = Part[o] + Part[SL + o] + Part[2 * SL + o] + Part[3 * SL + o];
+    Slab[o] = (r < bw && c < mc) ? bwr[e] - sum : T(0);
   }
   __syncthreads();
-  if (wv == 0) {
-#pragma unroll
-    for (int c = 0; c < MC; ++c) {
-      const T bwv = (r < bw && c < mc) ? B[(int64_t)(w0 + r) * ldb + c0 + c] : T(0);
-      const T sum = acc[c] + Part[0][r * MC + c] + Part[1][r * MC + c] + Part[2][r * MC + c];
-      Ys[0][r * MC + c] = (r < bw && c < mc) ? bwv - sum : T(0);
-    }
-  }
-  __syncthreads();
-  // X_w = inv(op(A)_ww) (B_w - sum): lane r, columns 4 wv .. 4 wv + 3
   T x[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll 8
   for (int k = 0; k < kNB; ++k) {
     const T d = Dl[r * S + k];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) x[e] += d * Ys[0][k * MC + 4 * wv + e];
+    for (int e = 0; e < 4; ++e) x[e] += d * Slab[k * MC + 4 * wv + e];
   }
+
+  // x -= G X_l for the dependency at position st, once its flag is raised
+  auto apply_g = [&](int st, const T* gs) -> bool {
+    if (st >= ready && !wait_for(st)) return false;
+    T y[4];
+    loady(st, y);
+    __syncthreads();  // every wave is done reading Slab and Part
+#pragma unroll
+    for (int e = 0; e < 4; ++e) Slab[tid + 256 * e] = y[e];
+    __syncthreads();
+    T gy[MC];
+#pragma unroll
+    for (int c = 0; c < MC; ++c) gy[c] = T(0);
+    const T* yk = Slab + KW * wv * MC;
+#pragma unroll
+    for (int kk = 0; kk < KW; ++kk) {
+      const T gv = gs[r * S + KW * wv + kk];
+#pragma unroll
+      for (int c = 0; c < MC; ++c) gy[c] += gv * yk[kk * MC + c];
+    }
+#pragma unroll
+    for (int c = 0; c < MC; ++c) Part[wv * SL + r * MC + c] = gy[c];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int o = r * MC + 4 * wv + e;
+      x[e] -= Part[o] + Part[SL + o] + Part[2 * SL + o] + Part[3 * SL + o];
+    }
+    return true;
+  };
+  if (p > 1 && !apply_g(l2, Gs[1])) return;
+  if (p > 0 && !apply_g(l1, Gs[0])) return;
 #pragma unroll
   for (int e = 0; e < 4; ++e)
     if (r < bw && 4 * wv + e < mc) B[(int64_t)(w0 + r) * ldb + c0 + 4 * wv + e] = x[e];

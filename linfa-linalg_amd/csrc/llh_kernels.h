@@ -192,11 +192,17 @@ hipError_t launch_convert_s2h(int rows, int cols, const float* X, int64_t ldx, _
                               hipStream_t s);
 // s[i] = 1 / sqrt(A[i][i]) for a positive finite diagonal entry, else 1.
 hipError_t launch_diag_rsqrt(int n, const double* A, int64_t lda, double* s_out, hipStream_t s);
+// R = B - A X, A symmetric from its lower triangle.  scratch (optional, in
+// doubles): room for the split partial sums; symm_scratch_elems(n, m) elements
+// give the most parallel split, less (or none) gives fewer splits.
 hipError_t launch_symm_residual(int n, int m, const double* A, int64_t lda, const double* X, int64_t ldx,
-                                const double* B, int64_t ldb, double* R, int64_t ldr, hipStream_t s);
+                                const double* B, int64_t ldb, double* R, int64_t ldr, hipStream_t s,
+                                double* scratch = nullptr, int64_t scratch_elems = 0);
+int64_t symm_scratch_elems(int n, int m);
 // out = |A| w (w null: 1), A symmetric from its lower triangle.
 hipError_t launch_symm_abs_rowsum(int n, const double* A, int64_t lda, double* out, hipStream_t s,
-                                  const double* w = nullptr);
+                                  const double* w = nullptr, double* scratch = nullptr,
+                                  int64_t scratch_elems = 0);
 // out[c] = max_r |X[r][c]| * rs[r] (inv: / rs[r]; rs null: 1).
 hipError_t launch_colmax(int rows, int cols, const double* X, int64_t ldx, double* out, hipStream_t s,
                          const double* rs = nullptr, bool inv = false);

@@ -78,7 +78,9 @@ hipError_t grow_buffer(llh_context* ctx, bool owns, llh_context::Buffer* borrowe
 llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error* err) {
   const int n = f->n;
   hipStream_t s = ctx->stream;
-  CHECK_HIP(grow_buffer(ctx, f->owns, &ctx->mixed_work, &f->work, &f->work_bytes, (size_t)(n + 2) * 8),
+  // work: [rowsum n | norm 2] [partial sums of the split row-sum walk]
+  const int64_t nsc = symm_scratch_elems(n, 1);
+  CHECK_HIP(grow_buffer(ctx, f->owns, &ctx->mixed_work, &f->work, &f->work_bytes, (size_t)(n + 2 + nsc) * 8),
             "work buffer");
   double* rowsum = static_cast<double*>(f->work);
   for (int attempt = 0; attempt < 2; ++attempt) {
@@ -95,7 +97,7 @@ llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error
     }
     // ||A||_inf (f16: ||diag(s) A diag(s)||_inf, the system it refines) from the
     // lower triangle, in f64
-    CHECK_HIP(launch_symm_abs_rowsum(n, f->a, f->lda, rowsum, s, sc), "row sums");
+    CHECK_HIP(launch_symm_abs_rowsum(n, f->a, f->lda, rowsum, s, sc, rowsum + n + 2, nsc), "row sums");
     CHECK_HIP(launch_colmax(n, 1, rowsum, 1, rowsum + n, s, sc), "norm");
     CHECK_HIP(hipMemcpyAsync(&f->anrm, rowsum + n, sizeof(double), hipMemcpyDeviceToHost, s), "norm readback");
     if (low_precision(f->prec)) {
@@ -152,19 +154,24 @@ llh_status solve_core(llh_context* ctx, llh_factor* f, size_t m_, const double* 
   llh_refine_info inf{0, 0, 0, (int32_t)f->prec, 0.0, f->anrm};
   const int max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 30;
   // work: [rowsum n | norm 2] [r: n*m f64] [colmax: 2m f64] [d: n*m f32]
+  // [partial sums of the split residual walk]
+  const int64_t nsc = symm_scratch_elems(n, m);
   const size_t off_r = (size_t)(n + 2) * 8, off_cm = off_r + (size_t)n * m * 8,
-               off_d = off_cm + (size_t)2 * m * 8, total = off_d + (size_t)n * m * 4;
+               off_d = off_cm + (size_t)2 * m * 8, off_p = (off_d + (size_t)n * m * 4 + 255) / 256 * 256,
+               total = off_p + (size_t)nsc * 8;
   CHECK_HIP(grow_buffer(ctx, f->owns, &ctx->mixed_work, &f->work, &f->work_bytes, total), "work buffer");
   char* w = static_cast<char*>(f->work);
   double* r = reinterpret_cast<double*>(w + off_r);
   double* cm = reinterpret_cast<double*>(w + off_cm);
   float* d = reinterpret_cast<float*>(w + off_d);
+  double* part = reinterpret_cast<double*>(w + off_p);
   SolveSync sync;
   CHECK_HIP(ctx_sync(ctx, n, m, &sync), "solve sync");
   const double cte = f->anrm * std::ldexp(1.0, -53) * std::sqrt((double)n);
   double hn[128];
   auto residual_check = [&](bool* conv) -> llh_status {
-    CHECK_HIP(launch_symm_residual(n, m, f->a, f->lda, x, (int64_t)ldx, b, (int64_t)ldb, r, m, s), "residual");
+    CHECK_HIP(launch_symm_residual(n, m, f->a, f->lda, x, (int64_t)ldx, b, (int64_t)ldb, r, m, s, part, nsc),
+              "residual");
     // the f16 factor refines diag(s) A diag(s) y = diag(s) b, y = x / s: its
     // test measures that system's residual and solution
     const double* rs = f->prec == LLH_PREC_F16 ? static_cast<const double*>(f->scale) : nullptr;

@@ -40,6 +40,9 @@ def test_f32_ir_matches_f64_oracle(n, lam):
     # both solutions are backward stable in f64: they agree to ~cond * eps64
     assert err <= max(1e-12, 50 * cond * np.finfo(np.float64).eps), (err, cond, info)
     assert info["residual"] <= 10 * np.sqrt(n) * np.finfo(np.float64).eps
+    # ||A||_inf from the split row-sum walk over the lower triangle
+    anorm = np.abs(a).sum(axis=1).max()
+    assert abs(info["anorm"] - anorm) <= 1e-13 * anorm, (info["anorm"], anorm)
 
 
 def test_f64_factor_and_reuse():
@@ -115,6 +118,10 @@ def test_f16_ir_matches_f64_oracle(n, lam):
     err = np.linalg.norm(x.cpu().numpy() - x_ref) / np.linalg.norm(x_ref)
     assert err <= 1e-12, (err, info)
     assert info["residual"] <= 10 * np.sqrt(n) * np.finfo(np.float64).eps
+    # the refinement measures the scaled system: ||S A S||_inf, S = diag(A)^-1/2
+    sc = 1.0 / np.sqrt(np.diag(a))
+    anorm = (np.abs(a) * sc[:, None] * sc[None, :]).sum(axis=1).max()
+    assert abs(info["anorm"] - anorm) <= 1e-13 * anorm, (info["anorm"], anorm)
 
 
 def test_f16_diagonal_scaling():
