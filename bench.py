@@ -24,7 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "linfa-linalg_amd"))
 
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "f16": 2500.0}  # dense MFMA peaks (f16: 2.5 PFLOP/s)
 OUTER = 512  # outer panel width of the library's blocked factor (rank of the trailing SYRK)
 PEAK_HBM_GBS = 8000.0
 
@@ -570,47 +570,75 @@ def main():
         log(f"bench: residual {check['residual']:.3e} backward error {check['backward_error']:.3e}")
         del l, res, al
 
-    # FP32-MFMA + f64 iterative refinement on the same A (A is not modified),
-    # then on A + (lambda - 1) I for smaller lambda (harder conditioning).
+    # Mixed precision on the same A (A is not modified): FP32-MFMA and
+    # FP16-MFMA (f32 accumulation, diagonally scaled) trailing updates with f64
+    # iterative refinement, then on A + (lambda - 1) I for smaller lambda
+    # (harder conditioning).  The timed runs are unprofiled and compared with an
+    # unprofiled f64 solve; a separate profiled run gives the kernel breakdown.
     mixed = None
+    mixed16 = None
     sweep = []
+    f64_clean_ms = None
     if not args.no_mixed:
         x32 = torch.zeros(n, m, dtype=torch.float64, device="cuda")
         x64 = copies[-1][1]
+        ca, cb = a0.clone(), rhs0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.solvec(ca, cb)
+        torch.cuda.synchronize()
+        f64_clean_ms = (time.perf_counter() - t0) * 1e3
+        del ca, cb
+        g64_clean = solve_flops(n, m) / (f64_clean_ms / 1e3) / 1e9
+        log(f"bench: unprofiled f64 solve {f64_clean_ms:.2f} ms ({g64_clean:.1f} GFLOP/s)")
+
+        def mixed_run(prec, lam):
+            info = dev.solve_mixed(a0, rhs0, x32, prec=prec, fallback=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                info = dev.solve_mixed(a0, rhs0, x32, prec=prec, fallback=True)
+            torch.cuda.synchronize()
+            msx = (time.perf_counter() - t0) * 1e3 / args.steps
+            gx = solve_flops(n, m) / (msx / 1e3) / 1e9
+            res = rhs0 - a0 @ x32
+            bres = float(res.norm() / (torch.linalg.matrix_norm(a0, ord=1) * x32.norm()))
+            del res
+            row = {"precision": prec, "lambda": lam, "cond_estimate": round((4.0 + lam) / lam, 1),
+                   "ms_per_step": round(msx, 3), "value": round(gx, 2), "unit": "GFLOP/s",
+                   "speedup_vs_f64": round(gx / g64_clean, 3), "refinement": info, "backward_residual": bres}
+            log(f"bench: {prec}+IR lambda={lam:g} {msx:.2f} ms/step, {gx:.1f} GFLOP/s, x{gx / g64_clean:.2f} vs "
+                f"f64, {info}, backward residual {bres:.2e}")
+            return row
+
+        def breakdown(row, prec, syrk_peak):
+            dev.set_profiling(True)
+            dev.solve_mixed(a0, rhs0, x32, prec=prec, fallback=True)
+            torch.cuda.synchronize()
+            st = dev.kernel_stats()
+            dev.set_profiling(False)
+            cnt_s, tot_s, fl_s, _ = st["trailing_syrk"]
+            row["solution_diff_vs_f64_gpu"] = float((x32 - x64).norm() / x64.norm())
+            row["syrk_tflops"] = round(fl_s / (tot_s / 1e3) / 1e12, 2) if tot_s > 0 else None
+            row["syrk_roofline_frac"] = (round(fl_s / (tot_s / 1e3) / 1e12 / PEAK_TFLOPS[syrk_peak], 4)
+                                         if tot_s > 0 else None)
+            row["kernels_ms_profiled_run"] = {k: round(v[1], 3) for k, v in st.items() if v[0]}
+
         lam_cur = args.lam
         for lam in [args.lam] + [v for v in (1e-2, 1e-4, 1e-6) if v != args.lam]:
             a0.diagonal().add_(lam - lam_cur)
             lam_cur = lam
-            log(f"bench: FP32+IR lambda={lam:g} warmup")
-            info = dev.solve_mixed(a0, rhs0, x32, prec="f32", fallback=True)
-            torch.cuda.synchronize()
-            dev.set_profiling(True)
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                info = dev.solve_mixed(a0, rhs0, x32, prec="f32", fallback=True)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            stats32 = dev.kernel_stats()
-            dev.set_profiling(False)
-            ms32 = (t1 - t0) * 1e3 / args.steps
-            g32 = solve_flops(n, m) / (ms32 / 1e3) / 1e9
-            res = rhs0 - a0 @ x32
-            bres = float(res.norm() / (torch.linalg.matrix_norm(a0, ord=1) * x32.norm()))
-            del res
-            row = {"lambda": lam, "cond_estimate": round((4.0 + lam) / lam, 1), "ms_per_step": round(ms32, 3),
-                   "value": round(g32, 2), "unit": "GFLOP/s", "speedup_vs_f64": round(g32 / gflops, 3),
-                   "refinement": info, "backward_residual": bres}
+            row = mixed_run("f32", lam)
             if lam == args.lam:
-                cnt32, tot32, fl32, _ = stats32["trailing_syrk"]
-                row["solution_diff_vs_f64_gpu"] = float((x32 - x64).norm() / x64.norm())
-                row["syrk_f32_tflops"] = round(fl32 / (tot32 / 1e3) / 1e12, 2) if tot32 > 0 else None
-                row["syrk_f32_roofline_frac"] = (round(fl32 / (tot32 / 1e3) / 1e12 / PEAK_TFLOPS["f32"], 4)
-                                                 if tot32 > 0 else None)
-                row["kernels"] = {k: round(v[1] / args.steps, 3) for k, v in stats32.items() if v[0]}
+                breakdown(row, "f32", "f32")
                 mixed = row
             sweep.append(row)
-            log(f"bench: FP32+IR lambda={lam:g} {ms32:.2f} ms/step, {g32:.1f} GFLOP/s, x{g32 / gflops:.2f} vs f64, "
-                f"{info}, backward residual {bres:.2e}")
+            if lam >= 1e-2:  # the f16 factor is for cond up to ~1e3
+                row16 = mixed_run("f16", lam)
+                if lam == args.lam:
+                    breakdown(row16, "f16", "f16")
+                    mixed16 = row16
+                sweep.append(row16)
         a0.diagonal().add_(args.lam - lam_cur)
         del x32
 
@@ -713,8 +741,10 @@ def main():
         "solve_roofline_frac_f64": round(gflops / 1e3 / PEAK_TFLOPS["f64"], 4),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "f64_unprofiled_ms": round(f64_clean_ms, 3) if f64_clean_ms else None,
         "fp32_ir": mixed,
-        "fp32_ir_lambda_sweep": sweep,
+        "fp16_ir": mixed16,
+        "mixed_lambda_sweep": sweep,
         "c2": c2,
         "c1": c1,
         "batched": batched,
