@@ -242,10 +242,11 @@ __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_ker
   const int m0 = tm * BM, n0 = tn * BN;
   if (TRI && m0 + BM - 1 < n0) return;
 
-  // C prefetch for the f64 trailing SYRK on 256 x 128 tiles (measured: it
-  // slows the f32 and f16 tiles, and the 128 x 128 f64 tile has no registers
-  // left)
-  constexpr bool PRE = SQ && BM == 256 && sizeof(T) == 8;
+  // C prefetch for the trailing SYRK: f64 on 256 x 128 tiles (the 128 x 128
+  // f64 tile has no registers left), f32 C (f32 or f16 operands) on 128 x 128
+  // tiles.  Measured at n = 32256, k = 512: f64 53 -> 57, f32 88 -> 107, f16
+  // 199 -> 348 TFLOP/s.
+  constexpr bool PRE = SQ && ((BM == 256 && sizeof(T) == 8) || sizeof(T) == 4);
   nt_tile<T, BM, BN, WM, WN, TRI, VEC, PRE, TI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
 }
 
@@ -351,7 +352,7 @@ hipError_t launch_syrk_lower_h(int n, int k, float alpha, const _Float16* A, int
   if (k < 0) k = 0;
   // 128 x 128 tiles: two workgroups per CU (72 KB of LDS each) hide more of the
   // operand latency than one 256 x 128 workgroup, measured 200 against 168
-  // TFLOP/s at n = 32256, k = 512
+  // TFLOP/s at n = 32256, k = 512 (348 with the C prefetch)
   if (variant == 1)
     return launch_nt<float, 256, 128, 4, 2, true, true, _Float16>(n, n, k, alpha, A, lda, A, lda, beta, C, ldc,
                                                                   gate, s);

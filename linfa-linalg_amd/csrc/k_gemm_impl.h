@@ -162,23 +162,35 @@ __device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const TI* 
   const T cscale = pre ? beta / alpha : T(0);
   acc_t cbuf[CPS];
   const int ci0 = m0 + wm0 + MF::row(lane, 0), cj0 = n0 + wn0 + fr;
-  const T* cbase = C + (int64_t)ci0 * ldc + cj0;
+  // The loads are unconditional, from indices clamped into the matrix (an
+  // entry above the diagonal of a square C is still C's memory); the entries
+  // outside the tile's part of C are dropped when folded in.  Masked loads here
+  // made the compiler zero cbuf first and wait for every outstanding load,
+  // the next slab's operands included, before the MFMAs.
   auto cload = [&](int g) {
 #pragma unroll
     for (int c = 0; c < CPS; ++c) {
       const int q = g * CPS + c, i = q / FN, j = q % FN;  // uniform
-      const int gj = cj0 + j * 16;
+      const int gj = min(cj0 + j * 16, N - 1);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int di = i * 16 + MF::kRowStep * r, gi = ci0 + di;
-        cbuf[c][r] = (gi < M && gj < N && (!TRI || gi >= gj)) ? cbase[(int64_t)di * ldc + j * 16] : T(0);
+        const int gi = min(ci0 + i * 16 + MF::kRowStep * r, M - 1);
+        cbuf[c][r] = C[(int64_t)gi * ldc + gj];
       }
     }
   };
   auto cadd = [&](int g) {
 #pragma unroll
     for (int q = 0; q < NF; ++q)
-      if (q / CPS == g) acc[q / FN][q % FN] += cscale * cbuf[q % CPS];
+      if (q / CPS == g) {
+        const int i = q / FN, j = q % FN, gj = cj0 + j * 16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gi = ci0 + i * 16 + MF::kRowStep * r;
+          const bool in = gi < M && gj < N && (!TRI || gi >= gj);
+          acc[i][j][r] += cscale * (in ? cbuf[q % CPS][r] : T(0));
+        }
+      }
   };
 
   load(0);
