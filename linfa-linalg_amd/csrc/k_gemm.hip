@@ -243,10 +243,12 @@ __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_ker
   if (TRI && m0 + BM - 1 < n0) return;
 
   // C prefetch for the trailing SYRK: f64 on 256 x 128 tiles (the 128 x 128
-  // f64 tile has no registers left), f32 C (f32 or f16 operands) on 128 x 128
-  // tiles.  Measured at n = 32256, k = 512: f64 53 -> 57, f32 88 -> 107, f16
-  // 199 -> 348 TFLOP/s.
-  constexpr bool PRE = SQ && ((BM == 256 && sizeof(T) == 8) || sizeof(T) == 4);
+  // f64 tile has no registers left) and f16 operands on 128 x 128 tiles.
+  // Measured at n = 32256, k = 512: f64 53 -> 57, f16 199 -> 348 TFLOP/s.  Not
+  // for f32 operands (88 -> 107 TFLOP/s): folding C into the f32 sum mid-way
+  // costs the accuracy that refinement then pays for (11 instead of 3 steps at
+  // cond ~4e6), and holding C to the end does not fit the registers.
+  constexpr bool PRE = SQ && ((BM == 256 && sizeof(T) == 8) || sizeof(TI) == 2);
   nt_tile<T, BM, BN, WM, WN, TRI, VEC, PRE, TI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
 }
 
