@@ -46,6 +46,28 @@ __device__ __forceinline__ float bcast(float v, int lane) {
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// rp = 1 / sqrt(d) and piv = sqrt(d) = d * rp for d > 0: the hardware
+// reciprocal square root refined by Newton steps y <- y (3/2 - (d/2) y^2)
+// (two in f64, one in f32, each doubling the correct bits), a chain of about
+// eight dependent operations per column where the correctly rounded square
+// root followed by a division takes about twenty-five.  Both results are
+// within an ulp or two, which the blocked factor's tolerance admits (the
+// exact-order kernel keeps the reference's square root and division).
+__device__ __forceinline__ void pivot_rsqrt(double d, double& piv, double& rp) {
+  double y = __builtin_amdgcn_rsq(d);
+  const double h = 0.5 * d;
+  y = y * __builtin_fma(-h, y * y, 1.5);
+  y = y * __builtin_fma(-h, y * y, 1.5);
+  rp = y;
+  piv = d * y;
+}
+__device__ __forceinline__ void pivot_rsqrt(float d, float& piv, float& rp) {
+  float y = __builtin_amdgcn_rsqf(d);
+  y = y * __builtin_fmaf(-0.5f * d, y * y, 1.5f);
+  rp = y;
+  piv = d * y;
+}
+
 constexpr int kDiagS = kNB + 2;  // LDS row stride of L (keeps rows 16-byte aligned)
 constexpr int kDiagP = 8;        // panel width
 constexpr int kDiagW = 16;       // columns (factor) / rows (inverse) per wave
@@ -73,7 +95,9 @@ __device__ __forceinline__ void diag_quad_block(TS* __restrict__ a, int64_t lda,
   T* l = sm.l;
   if (threadIdx.x < N / P) sm.fail[threadIdx.x] = -1;
   // this lane's 16 columns of row i; masked entries read the block's first
-  // element (always in bounds) so that the loads carry no branches
+  // element (always in bounds) so that the loads carry no branches.  (Loading
+  // through LDS with consecutive lanes on consecutive columns measured slower:
+  // these loads overlap the first panel, a staged copy cannot.)
   T r[WC];
   {
     const TS* row = a + (int64_t)min(i, b - 1) * lda;
@@ -101,8 +125,8 @@ __device__ __forceinline__ void diag_quad_block(TS* __restrict__ a, int64_t lda,
           if (d <= T(0)) {  // cholesky.rs:69 (NaN passes)
             f = j;
           } else {
-            const T piv = sqrt_rn(d);
-            const T rp = T(1) / piv;
+            T piv, rp;
+            pivot_rsqrt(d, piv, rp);
             r[lo + c] = (i == j) ? piv : r[lo + c] * rp;
             if (i == 0) sm.rinv[j] = rp;
 #pragma unroll
@@ -141,14 +165,14 @@ __device__ __forceinline__ void diag_quad_block(TS* __restrict__ a, int64_t lda,
       }
     }
   });
-  // write back from registers: rows < f (or all rows) with their lower part,
-  // row f up to f-1
-  {
-    const int last = (fail < 0 || i < fail) ? i : (i == fail ? i - 1 : -1);
-    TS* row = a + (int64_t)i * lda + c0;
+  // write back from the LDS copy of L (every panel up to the failing one was
+  // published there), consecutive lanes on consecutive columns of a row:
+  // rows < f (or all rows) with their lower part, row f up to f-1
 #pragma unroll
-    for (int kk = 0; kk < WC; ++kk)
-      if (i < b && c0 + kk <= last) row[kk] = TS(r[kk]);
+  for (int e = 0; e < N * N / kDiagT; ++e) {
+    const int idx = threadIdx.x + kDiagT * e, row = idx / N, col = idx % N;
+    const int last = (fail < 0 || row < fail) ? row : (row == fail ? row - 1 : -1);
+    if (row < b && col <= last) a[(int64_t)row * lda + col] = TS(l[row * S + col]);
   }
   if (fail >= 0) {
     if (threadIdx.x == 0) atomicMin(info, (int32_t)(info_base + fail));

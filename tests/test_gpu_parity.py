@@ -251,10 +251,16 @@ def test_property_solve_triangular(uplo, n, m, data):
     a[np.diag_indices(n)] = d
     x = data.draw(matrix(n, m))
     b = a @ x
+    # The reference's 1e-4 bound on |out - x| holds unless the drawn system is
+    # so ill-conditioned that the reference algorithm's own result misses it too
+    # (hypothesis finds such draws: diagonal 1, off-diagonal entries near 100);
+    # then the bound is a small multiple of the oracle's error.
+    ref = orc.solve_triangular(np.array(a), b, uplo)
+    tol = max(1e-4, 10 * np.abs(ref - x).max())
     out = llh.solve_triangular(a, b, uplo)
-    assert np.abs(out - x).max() <= 1e-4
+    assert np.abs(out - x).max() <= tol
     out2 = llh.solve_triangular_inplace(a, np.array(b), uplo)
-    assert np.abs(out2 - x).max() <= 1e-4
+    assert np.abs(out2 - x).max() <= tol
 
 
 # --- device-resident forms (llh_dev_*) through torch tensors ----------------------
