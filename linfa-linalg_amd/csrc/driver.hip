@@ -25,43 +25,50 @@ namespace llh {
 
 namespace {
 
-// Factors the outer panel of columns [k0, k0 + kb) on stream st, kNB columns at
-// a time: the diagonal block in LDS (with its inverse), the rows below it as one
-// MFMA product with that inverse, then the rest of the outer panel.  Everything
-// left of k0 has already been applied to these columns.
+// Factors the outer panel of columns [k0, k0 + kb) on stream st, everything
+// left of k0 having been applied to these columns.  The panel is split in two
+// halves recursively down to kNB columns: factor the left half, update the
+// right half's columns with it (one GEMM, rank = the left half's width, lower
+// part of the top square), factor the right half.  A kNB-wide block is the
+// diagonal block in LDS (with its inverse), then the rows below it as one MFMA
+// product with that inverse.  Against a kNB-at-a-time right-looking sweep the
+// launches are as many, but each column of the panel is read and written by
+// log2(kb / kNB) updates instead of up to kb / kNB - 1, and the updates have rank
+// 64 .. 256 instead of 64.
 template <typename T>
 hipError_t factor_panel(T* a, int64_t lda, int n, int k0, int kb, int32_t* info, T* dinv, hipStream_t st,
                         Profiler* prof, bool diag_f64) {
   const double es = (double)sizeof(T);
-  for (int k = k0; k < k0 + kb; k += kNB) {
-    const int b = std::min(kNB, k0 + kb - k);
-    T* akk = a + (int64_t)k * lda + k;
-    const double db = b;
-    T* dk = dinv + (int64_t)(k / kNB) * kNB * kNB;
-    if (diag_f64 && sizeof(T) == 4) {
-      LLH_PROF_ON(st, kDiagFactor, 2.0 * db * db * db / 3.0, es * db * db * 2.0,
-                  (launch_diag_fast<T, double>(akk, lda, b, dk, info, k, info, st)));
-    } else {
-      LLH_PROF_ON(st, kDiagFactor, 2.0 * db * db * db / 3.0, es * db * db * 2.0,
-                  (launch_diag_fast<T>(akk, lda, b, dk, info, k, info, st)));
-    }
-    const int below = n - k - b;
-    if (below <= 0) continue;
-    T* panel = akk + (int64_t)b * lda;  // rows k+b.., columns k..k+b
-    // L21 = A21 * L11^-T as one MFMA product with the inverse, in place (each
-    // workgroup reads all of its rows' k before writing them).
-    LLH_PROF_ON(st, kPanelTrsm, (double)below * db * db * 2.0, es * ((double)below * db * 2.0 + db * db),
-                (launch_gemm<T>(false, true, false, below, b, b, T(1), panel, lda, dk, kNB, T(0), panel, lda, info,
-                                st)));
-    const int cols = k0 + kb - (k + b);
-    if (cols > 0) {
-      // rows below x cols, lower part of the top cols x cols square only
-      const double outs = (double)below * cols - (double)cols * (cols - 1) / 2.0;
-      LLH_PROF_ON(st, kPanelUpdate, 2.0 * outs * db, es * (2.0 * outs + ((double)below + cols) * db),
-                  (launch_gemm<T>(false, true, true, below, cols, b, T(-1), panel, lda, panel, lda, T(1), panel + b,
-                                  lda, info, st)));
-    }
+  if (kb > kNB) {
+    const int h = (kb / kNB + 1) / 2 * kNB;  // left half: a whole number of blocks, at least one
+    LLH_TRY(factor_panel<T>(a, lda, n, k0, h, info, dinv, st, prof, diag_f64));
+    const int below = n - k0 - h, cols = kb - h;  // rows k0+h.., columns k0+h .. k0+kb
+    T* left = a + (int64_t)(k0 + h) * lda + k0;    // L of rows k0+h.., columns k0 .. k0+h
+    const double outs = (double)below * cols - (double)cols * (cols - 1) / 2.0, dh = h;
+    LLH_PROF_ON(st, kPanelUpdate, 2.0 * outs * dh, es * (2.0 * outs + ((double)below + cols) * dh),
+                (launch_gemm<T>(false, true, true, below, cols, h, T(-1), left, lda, left, lda, T(1), left + h, lda,
+                                info, st)));
+    return factor_panel<T>(a, lda, n, k0 + h, cols, info, dinv, st, prof, diag_f64);
   }
+  const int k = k0, b = kb;
+  T* akk = a + (int64_t)k * lda + k;
+  const double db = b;
+  T* dk = dinv + (int64_t)(k / kNB) * kNB * kNB;
+  if (diag_f64 && sizeof(T) == 4) {
+    LLH_PROF_ON(st, kDiagFactor, 2.0 * db * db * db / 3.0, es * db * db * 2.0,
+                (launch_diag_fast<T, double>(akk, lda, b, dk, info, k, info, st)));
+  } else {
+    LLH_PROF_ON(st, kDiagFactor, 2.0 * db * db * db / 3.0, es * db * db * 2.0,
+                (launch_diag_fast<T>(akk, lda, b, dk, info, k, info, st)));
+  }
+  const int below = n - k - b;
+  if (below <= 0) return hipSuccess;
+  T* panel = akk + (int64_t)b * lda;  // rows k+b.., columns k..k+b
+  // L21 = A21 * L11^-T as one MFMA product with the inverse, in place (each
+  // workgroup reads all of its rows' k before writing them).
+  LLH_PROF_ON(st, kPanelTrsm, (double)below * db * db * 2.0, es * ((double)below * db * 2.0 + db * db),
+              (launch_gemm<T>(false, true, false, below, b, b, T(1), panel, lda, dk, kNB, T(0), panel, lda, info,
+                              st)));
   return hipSuccess;
 }
 
