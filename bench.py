@@ -25,7 +25,10 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "linfa-linalg_amd"))
 
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "f16": 2500.0}  # dense MFMA peaks (f16: 2.5 PFLOP/s)
-OUTER = 512  # outer panel width of the library's blocked factor (rank of the trailing SYRK)
+def outer_width(n):
+    """Outer panel width of the library's blocked f64 factor (the rank of its
+    trailing SYRK): 1024 from n = 24576 on, else 512."""
+    return 1024 if n >= 24576 else 512
 PEAK_HBM_GBS = 8000.0
 
 
@@ -409,12 +412,13 @@ def rocprof_summary(trace_dir, fetch_dir, write_dir, n, out_md, out_json, comman
                          "avg_kib": sum(vals) / len(vals), "bytes_per_launch": sum(vals) / len(vals) * 1024 * factor}
     traffic = counters["FETCH_SIZE"]["bytes_per_launch"] + counters["WRITE_SIZE"]["bytes_per_launch"]
     # algorithmic bytes of the trailing SYRKs of one factorization (outer panel
-    # width OUTER; with lookahead each SYRK leaves out the next panel's columns)
+    # width outer_width(n); with lookahead each SYRK leaves out the next panel's columns)
     alg = []
     k0 = 0
+    outer = outer_width(n)
     while k0 < n:
-        kb = min(OUTER, n - k0)
-        m3 = n - k0 - kb - min(OUTER, max(n - k0 - kb, 0))
+        kb = min(outer, n - k0)
+        m3 = n - k0 - kb - min(outer, max(n - k0 - kb, 0))
         if m3 > 0:
             alg.append(8.0 * (m3 * (m3 + 1) + m3 * kb))
         k0 += kb
@@ -555,7 +559,7 @@ def main():
                                  % (tr["fetch"]["factor"], tr["write"]["factor"], tr["traffic_over_algorithmic"], n))
                 if tr else None,
                 "kernel": "gemm_nt_kernel<double,256,128,4,2,TRI,SQ> (trailing SYRK, rank %d; 128x128 tiles once the "
-                          "trailing matrix is below 4096^2)" % OUTER,
+                          "trailing matrix is below 4096^2)" % outer_width(n),
                 "avg_launch_us": round(tot_ms / cnt * 1e3, 2), "launches_per_step": cnt // args.steps,
                 "algorithmic_tflop_per_step": round(fl / args.steps / 1e12, 4)}
 

@@ -90,8 +90,8 @@ hipError_t trailing_syrk(int m, int kb, const T* p, int64_t lda, T* c, int32_t* 
 }  // namespace
 
 // Right-looking blocked Cholesky of the lower triangle (the blocked equivalent
-// of cholesky.rs:54-74).  Each kNBO-wide outer panel is factored by
-// factor_panel; then one rank-kNBO SYRK on MFMA updates the trailing matrix.
+// of cholesky.rs:54-74).  Each outer panel (outer_width(n) columns) is factored
+// by factor_panel; then one SYRK of that rank on MFMA updates the trailing matrix.
 // With lookahead, the trailing update of panel k is split: the next panel's
 // columns are updated and that panel factored on the side stream while the SYRK
 // of the remaining trailing matrix runs on s.  Matrices no larger than the LDS
@@ -112,12 +112,13 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
     LLH_TRY(launch_potrf_exact_single<T>(a, lda, n, info, 0, nullptr, s));
     if (dinv != nullptr) LLH_TRY(launch_trtri_blocks<T>(n, a, lda, false, false, dinv, info, s));
   } else {
-    LLH_TRY(factor_panel<T>(a, lda, n, 0, std::min(kNBO, n), info, dinv, s, prof, diag_f64));
-    for (int k0 = 0; k0 < n; k0 += kNBO) {
-      const int kb = std::min(kNBO, n - k0);
+    const int nbo = outer_width(n, half != nullptr);
+    LLH_TRY(factor_panel<T>(a, lda, n, 0, std::min(nbo, n), info, dinv, s, prof, diag_f64));
+    for (int k0 = 0; k0 < n; k0 += nbo) {
+      const int kb = std::min(nbo, n - k0);
       const int m2 = n - k0 - kb;
       if (m2 <= 0) break;
-      const int kb2 = std::min(kNBO, m2);  // width of the next panel
+      const int kb2 = std::min(nbo, m2);  // width of the next panel
       T* p = a + (int64_t)(k0 + kb) * lda + k0;  // L21: rows k0+kb.., columns k0..k0+kb
       T* c = p + kb;                              // trailing matrix at (k0+kb, k0+kb)
       // The trailing SYRK's algorithmic work: outs outputs of the lower

@@ -13,12 +13,19 @@ namespace llh {
 constexpr int32_t kNoFail = 0x7fffffff;
 
 // Block sizes of the blocked right-looking POTRF: kNB is the width of the
-// diagonal block factored in LDS and of the panel solved row by row; kNBO is
-// the rank of the trailing SYRK update on MFMA (512: each output tile then runs
-// 32 MFMA k-slabs per read-modify-write of C; measured 47.6 against 43.2 TF/s
-// for 256, with the larger panel chain hidden by the lookahead stream).
+// diagonal block factored in LDS and of the panel solved row by row; the outer
+// panel width, the rank of the trailing SYRK update on MFMA, is kNBO, or
+// kNBO_LARGE for an order of at least kNBO_LARGE_N when the SYRK runs on f64 or
+// f32 operands (outer_width()).  Wider panels give each output tile more MFMA
+// k-slabs per read-modify-write of C but a longer panel chain.  Measured solves:
+// n = 32768 f64 229 / 220 / 223 / 226 ms at 512 / 1024 / 1536 / 2048; n = 16384
+// f64 41.6 / 42.5 ms at 512 / 1024; the f16-operand factor, whose SYRK is fast and
+// whose panel chain sets the pace, 87 / 95 ms at 512 / 1024 (n = 32768).
 constexpr int kNB = 64;
 constexpr int kNBO = 512;
+constexpr int kNBO_LARGE = 1024;
+constexpr int kNBO_LARGE_N = 24576;
+inline int outer_width(int n, bool half_syrk) { return (!half_syrk && n >= kNBO_LARGE_N) ? kNBO_LARGE : kNBO; }
 
 // Right-hand-side columns per workgroup of the one-launch triangular solve, and
 // the bound on its flag waits (2^22 polls: well over 0.1 s, far above any real wait).

@@ -35,7 +35,7 @@ struct Lookahead {
 // f64 arithmetic, the mixed-precision form.
 // la (optional): lookahead streams; the whole factor is still complete in
 // stream order on s.
-// half (f32 storage only, optional): scratch of n * kNBO f16 values; each
+// half (f32 storage only, optional): scratch of n * outer_width(n, true) f16 values; each
 // trailing SYRK then rounds its panel to f16 there and multiplies on the f16
 // MFMA with f32 accumulation (the half-precision mixed factor).
 template <typename T>

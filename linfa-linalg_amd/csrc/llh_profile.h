@@ -11,7 +11,7 @@
 namespace llh {
 
 enum KernelClass : int {
-  kTrailingSyrk = 0,  // rank-kNBO trailing update of the POTRF (MFMA)
+  kTrailingSyrk = 0,  // trailing update of the POTRF, rank = the outer panel width (MFMA)
   kPanelUpdate = 1,   // inner-panel update of the POTRF (MFMA)
   kDiagFactor = 2,    // diagonal-block factor in LDS
   kPanelTrsm = 3,     // panel row solve

@@ -309,6 +309,32 @@ def test_dev_solvec_and_potrf(n):
     assert np.all(np.triu(c, 1) == 0) and np.array_equal(c, np.tril(fac))
 
 
+def test_dev_solvec_wide_outer_panels():
+    """From n = 24576 on the f64 factor uses 1024-wide outer panels (its SYRK's
+    rank); n = 24657 also leaves a ragged last panel and block.  Too large for
+    the oracle: the size-independent checks are the backward residual of the
+    solve and L L^T = A on a sample of rows, computed on the GPU in f64."""
+    import torch
+    from linfa_linalg_hip import device as dev
+    n, m = 24657, 8
+    g = torch.Generator(device="cuda").manual_seed(11)
+    bm = torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g).sub_(0.5)
+    a = torch.matmul(bm, bm.T).div_(n)
+    del bm
+    a.diagonal().add_(1.0)
+    rhs = torch.randn(n, m, dtype=torch.float64, device="cuda", generator=g)
+    fac, x = a.clone(), rhs.clone()
+    dev.solvec(fac, x)
+    res = (torch.matmul(a, x) - rhs).norm() / (a.norm() * x.norm())
+    assert float(res) <= 1e-15, float(res)
+    rows = torch.tensor([0, 1, 511, 512, 1023, 1024, 1025, 12345, 24575, 24576, n - 2, n - 1], device="cuda")
+    l = torch.tril(fac)
+    lr = l[rows]
+    err = (torch.matmul(lr, l.T) - a[rows]).abs().max() / a.abs().max()
+    assert float(err) <= 1e-14, float(err)
+    assert torch.equal(torch.triu(fac, 1), torch.triu(a, 1))  # dirty: upper untouched
+
+
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-4)])
 @pytest.mark.parametrize("n", [5, 70, 300])
 def test_solve_triangular_system_matches_oracle(dtype, tol, n):
