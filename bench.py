@@ -525,7 +525,9 @@ def main():
         dev.solvec(*copies[i])
     torch.cuda.synchronize()
 
-    dev.set_profiling(True)
+    # The timed steps record live events for the trailing SYRK only (the
+    # roofline kernel); an extra profiled step gives the other classes.
+    dev.set_profiling(True, classes=["trailing_syrk"])
     torch.cuda.synchronize()
     log(f"bench: timing {args.steps} steps")
     t0 = time.perf_counter()
@@ -535,15 +537,22 @@ def main():
     t1 = time.perf_counter()
     stats = dev.kernel_stats()
     dev.set_profiling(False)
+    ca, cb = a0.clone(), rhs0.clone()
+    dev.set_profiling(True)
+    dev.solvec(ca, cb)
+    torch.cuda.synchronize()
+    stats_all = dev.kernel_stats()
+    dev.set_profiling(False)
+    del ca, cb
     ms = (t1 - t0) * 1e3 / args.steps
     gflops = solve_flops(n, m) / (ms / 1e3) / 1e9
     log(f"bench: {ms:.2f} ms/step, {gflops:.1f} GFLOP/s")
 
     kernels = {}
-    for name, (cnt, tot_ms, fl, by) in stats.items():
+    for name, (cnt, tot_ms, fl, by) in stats_all.items():
         if cnt == 0:
             continue
-        kernels[name] = {"launches_per_step": cnt // args.steps, "ms_per_step": round(tot_ms / args.steps, 3),
+        kernels[name] = {"launches_per_step": cnt, "ms_per_step": round(tot_ms, 3),
                          "avg_launch_us": round(tot_ms / cnt * 1e3, 2),
                          "tflops": round(fl / (tot_ms / 1e3) / 1e12, 2) if tot_ms > 0 else None,
                          "gbs": round(by / (tot_ms / 1e3) / 1e9, 1) if tot_ms > 0 else None}
@@ -755,6 +764,7 @@ def main():
         "serving": serving,
         "lobpcg_qr": next_rows,
         "kernels": kernels,
+        "kernels_note": "one extra profiled solve, every kernel class; the timed steps record the trailing SYRK only",
         "check": check,
     }
     print(json.dumps(out), flush=True)

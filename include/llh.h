@@ -79,8 +79,8 @@ const char* llh_status_string(int32_t code);
  * with HIP events on the context's stream and add the launch's algorithmic
  * flops and bytes.  Enabling clears earlier records. */
 typedef enum llh_kernel_class {
-  LLH_KC_TRAILING_SYRK = 0, /* rank-256 trailing update of the POTRF (MFMA) */
-  LLH_KC_PANEL_UPDATE = 1,  /* update inside the 256-wide outer panel (MFMA) */
+  LLH_KC_TRAILING_SYRK = 0, /* trailing update of the POTRF, rank = the outer panel width (MFMA) */
+  LLH_KC_PANEL_UPDATE = 1,  /* updates inside the outer panel and of the look-ahead panel (MFMA) */
   LLH_KC_DIAG_FACTOR = 2,   /* 64 x 64 diagonal-block factor in LDS */
   LLH_KC_PANEL_TRSM = 3,    /* panel row solve X := X L11^-T */
   LLH_KC_SOLVE_GEMM = 4,    /* block update of the triangular solves (MFMA) */
@@ -95,6 +95,10 @@ typedef struct llh_kernel_stats {
 } llh_kernel_stats;
 
 llh_status llh_context_set_profiling(llh_context* ctx, int enable, llh_error* err);
+/* As llh_context_set_profiling(ctx, 1, err), but records only the classes whose
+ * bit (1 << llh_kernel_class) is set in class_mask (0: off): fewer events on
+ * the streams when only some classes are wanted. */
+llh_status llh_context_set_profiling_classes(llh_context* ctx, uint32_t class_mask, llh_error* err);
 /* Synchronizes the context's stream, then sums the records of one class. */
 llh_status llh_context_kernel_stats(llh_context* ctx, int kernel_class, llh_kernel_stats* out,
                                     llh_error* err);

@@ -702,6 +702,14 @@ llh_status llh_context_set_profiling(llh_context* ctx, int enable, llh_error* er
   }
   ctx->prof.recs.clear();
   ctx->prof.on = enable != 0;
+  ctx->prof.mask = ~0u;
+  return ok(err);
+}
+
+llh_status llh_context_set_profiling_classes(llh_context* ctx, uint32_t class_mask, llh_error* err) {
+  llh_status st = llh_context_set_profiling(ctx, class_mask != 0, err);
+  if (st != LLH_OK) return st;
+  ctx->prof.mask = class_mask;
   return ok(err);
 }
 

@@ -22,6 +22,7 @@ enum KernelClass : int {
 
 struct Profiler {
   bool on = false;
+  uint32_t mask = ~0u;  // classes recorded while on
   struct Rec {
     hipEvent_t start, stop;
     int cls;
@@ -43,7 +44,7 @@ struct Profiler {
   }
   // Records the start event of one launch; returns its index or -1 when off.
   int begin(int cls, double flops, double bytes, hipStream_t s) {
-    if (!on) return -1;
+    if (!on || !(mask & (1u << cls))) return -1;
     Rec r{get(), get(), cls, flops, bytes};
     (void)hipEventRecord(r.start, s);
     recs.push_back(r);

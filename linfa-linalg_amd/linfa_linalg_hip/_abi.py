@@ -88,6 +88,7 @@ def _declare(lib):
     lib.llh_status_string.argtypes = [ctypes.c_int32]
     lib.llh_status_string.restype = ctypes.c_char_p
     lib.llh_context_set_profiling.argtypes = [_vp, _int, _err]
+    lib.llh_context_set_profiling_classes.argtypes = [_vp, ctypes.c_uint32, _err]
     lib.llh_context_kernel_stats.argtypes = [_vp, _int, ctypes.POINTER(KernelStats), _err]
     lib.llh_factor_create.argtypes = [_vp, _int, _sz, _vp, _sz, ctypes.c_int32, ctypes.POINTER(_vp), _err]
     lib.llh_factor_solve.argtypes = [_vp, _vp, _sz, _vp, _sz, _vp, _sz, ctypes.POINTER(RefineOpts),

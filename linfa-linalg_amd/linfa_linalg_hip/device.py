@@ -102,9 +102,17 @@ def apply_constraints(v, cholesky_yy, y):
     return v
 
 
-def set_profiling(enable):
+def set_profiling(enable, classes=None):
+    """Live kernel timing on or off; classes (names of _abi.KERNEL_CLASSES)
+    restricts the recorded launches to those classes."""
     c = context()
     err = _abi.LlhError()
+    if enable and classes is not None:
+        mask = 0
+        for name in classes:
+            mask |= 1 << _abi.KERNEL_CLASSES[name]
+        _raise(c._lib.llh_context_set_profiling_classes(c.handle, mask, ctypes.byref(err)), err)
+        return
     _raise(c._lib.llh_context_set_profiling(c.handle, int(bool(enable)), ctypes.byref(err)), err)
 
 
