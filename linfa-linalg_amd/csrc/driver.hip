@@ -131,15 +131,31 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
         LLH_TRY(factor_panel<T>(a, lda, n, k0 + kb, kb2, info, dinv, s, prof, diag_f64));
         continue;
       }
+      // With half, the whole panel (m2 x kb) is rounded to f16 once, before the
+      // fork: the look-ahead update and the SYRK both read the copy.
+      bool half_ready = false;
+      if constexpr (sizeof(T) == 4) {
+        if (half != nullptr) {
+          LLH_TRY(launch_convert_s2h(m2, kb, p, lda, half, kb, s));
+          half_ready = true;
+        }
+      }
       LLH_TRY(hipEventRecord(la->fork, s));
       LLH_TRY(hipStreamWaitEvent(la->side, la->fork, 0));
       {
         // next panel's columns (m2 rows x kb2 columns, lower part of the top
         // square), then its factor, on the side stream
         const double outs = (double)m2 * kb2 - (double)kb2 * (kb2 - 1) / 2.0;
-        LLH_PROF_ON(la->side, kPanelUpdate, 2.0 * outs * kb, es * (2.0 * outs + ((double)m2 + kb2) * kb),
-                    (launch_gemm<T>(false, true, true, m2, kb2, kb, T(-1), p, lda, p, lda, T(1), c, lda, info,
-                                    la->side)));
+        if (half_ready) {
+          if constexpr (sizeof(T) == 4)
+            LLH_PROF_ON(la->side, kPanelUpdate, 2.0 * outs * kb, 4.0 * 2.0 * outs + 2.0 * ((double)m2 + kb2) * kb,
+                        (launch_gemm_nt_tri_h(m2, kb2, kb, -1.0f, half, kb, half, kb, 1.0f, c, lda, info,
+                                              la->side)));
+        } else {
+          LLH_PROF_ON(la->side, kPanelUpdate, 2.0 * outs * kb, es * (2.0 * outs + ((double)m2 + kb2) * kb),
+                      (launch_gemm<T>(false, true, true, m2, kb2, kb, T(-1), p, lda, p, lda, T(1), c, lda, info,
+                                      la->side)));
+        }
         LLH_TRY(factor_panel<T>(a, lda, n, k0 + kb, kb2, info, dinv, la->side, prof, diag_f64));
       }
       LLH_TRY(hipEventRecord(la->join, la->side));
@@ -147,8 +163,15 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
       if (m3 > 0) {
         const double outs = (double)m3 * (m3 + 1) / 2.0;
         T* p3 = p + (int64_t)kb2 * lda;
-        LLH_PROF(kTrailingSyrk, 2.0 * outs * kb, es * 2.0 * outs + pes * (double)m3 * kb,
-                 (trailing_syrk<T>(m3, kb, p3, lda, c + (int64_t)kb2 * lda + kb2, info, half, s)));
+        T* c3 = c + (int64_t)kb2 * lda + kb2;
+        if (half_ready) {
+          if constexpr (sizeof(T) == 4)
+            LLH_PROF(kTrailingSyrk, 2.0 * outs * kb, es * 2.0 * outs + 2.0 * (double)m3 * kb,
+                     (launch_syrk_lower_h(m3, kb, -1.0f, half + (int64_t)kb2 * kb, kb, 1.0f, c3, lda, info, s)));
+        } else {
+          LLH_PROF(kTrailingSyrk, 2.0 * outs * kb, es * 2.0 * outs + pes * (double)m3 * kb,
+                   (trailing_syrk<T>(m3, kb, p3, lda, c3, info, half, s)));
+        }
       }
       LLH_TRY(hipStreamWaitEvent(s, la->join, 0));
     }

@@ -362,6 +362,19 @@ hipError_t launch_syrk_lower_h(int n, int k, float alpha, const _Float16* A, int
                                                                 s);
 }
 
+hipError_t launch_gemm_nt_tri_h(int M, int N, int K, float alpha, const _Float16* A, int64_t lda,
+                                const _Float16* B, int64_t ldb, float beta, float* C, int64_t ldc,
+                                const int32_t* gate, hipStream_t s) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (K < 0) K = 0;
+  const int64_t t128 = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
+  if (N <= 64 || t128 < 128)
+    return launch_nt<float, 64, 64, 2, 2, true, false, _Float16>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate,
+                                                                  s);
+  return launch_nt<float, 128, 128, 2, 2, true, false, _Float16>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate,
+                                                                  s);
+}
+
 template hipError_t launch_syrk_lower<double>(int, int, double, const double*, int64_t, double, double*, int64_t,
                                               const int32_t*, hipStream_t);
 template hipError_t launch_syrk_lower<float>(int, int, float, const float*, int64_t, float, float*, int64_t,

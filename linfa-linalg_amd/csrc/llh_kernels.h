@@ -153,6 +153,11 @@ hipError_t launch_syrk_lower(int n, int k, T alpha, const T* A, int64_t lda, T b
 // 128 x 128, 1: 256 x 128).
 hipError_t launch_syrk_lower_h(int n, int k, float alpha, const _Float16* A, int64_t lda, float beta, float* C,
                                int64_t ldc, const int32_t* gate, hipStream_t s, int variant = 0);
+// C (M x N, f32, lower part of the top N x N square only) = beta C + alpha A B^T
+// with f16 operands: the look-ahead panel's update of the same factor.
+hipError_t launch_gemm_nt_tri_h(int M, int N, int K, float alpha, const _Float16* A, int64_t lda,
+                                const _Float16* B, int64_t ldb, float beta, float* C, int64_t ldc,
+                                const int32_t* gate, hipStream_t s);
 
 // k_trsm.hip: panel solve X := X * L^-T (X is m x b, L is b x b lower, b <= kNB).
 template <typename T>
