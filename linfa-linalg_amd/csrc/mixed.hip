@@ -106,8 +106,8 @@ llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error
       if (f->prec == LLH_PREC_F16) {
         void* hp = nullptr;
         size_t hb = 0;
-        CHECK_HIP(grow_buffer(ctx, false, &ctx->half_panel, &hp, &hb, (size_t)n * outer_width(n, true) * sizeof(_Float16)),
-                  "f16 panel buffer");
+        const size_t want = (size_t)n * outer_width(n, true) * sizeof(_Float16);
+        CHECK_HIP(grow_buffer(ctx, false, &ctx->half_panel, &hp, &hb, want), "f16 panel buffer");
         half = static_cast<_Float16*>(hp);
       }
       CHECK_HIP(launch_convert_d2s(n, n, f->a, f->lda, l, n, true, s, sc, sc), "convert");
