@@ -387,7 +387,10 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void trsm_persistent_k
 #pragma unroll
   for (int e = 0; e < 4; ++e)
     if (r < bw && 4 * wv + e < mc) B[(int64_t)(w0 + r) * ldb + c0 + 4 * wv + e] = x[e];
-  __threadfence();
+  // Each wave releases its own stores at agent scope (a release fence: the
+  // seq_cst __threadfence would also invalidate the L2 after every block);
+  // thread 0's release store of the flag follows the barrier.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (tid == 0) __hip_atomic_store(&flags[w * nq + q], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
