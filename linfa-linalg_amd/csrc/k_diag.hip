@@ -3,12 +3,13 @@
 // the block may too; small whole matrices keep the exact-order kernel).
 //
 // Four waves, lane i = row i in every wave; wave w holds columns
-// [16w, 16w + 16) of its row in registers.  The block is factored in panels of
-// 8 columns; panel p belongs to wave p/2, which factors it with the pivot and
-// each L[k, j] taken from their lanes by v_readlane (no barrier inside a panel)
-// and writes it to a row-major copy of L in LDS.  After one barrier every wave
-// applies the panel's rank-8 update to its own columns to the right of the
-// panel, so the trailing work is spread over the four waves.  The inverse is
+// [16w, 16w + 16) of its row in registers.  The block is factored in four
+// steps; in step w wave w factors its two 8-column panels with the pivot and
+// each L[k, j] taken from their lanes by v_readlane (no barrier inside a wave's
+// step, and no branch on the data: a failing pivot is recorded, not branched
+// on) and writes them to a row-major copy of L in LDS.  After one barrier every
+// later wave applies the step's rank-16 update to its own columns, so the
+// trailing work is spread over the four waves.  The inverse is
 // split the same way by rows: lane c holds column c of inv(L), wave w rows
 // [16w, 16w + 16); the wave owning a panel of 8 rows finishes them, publishes
 // them to a double-buffered LDS slab, and after a barrier the waves below take
@@ -78,7 +79,7 @@ struct DiagLds {
   T l[kNB * kDiagS];             // L (row-major), written a panel at a time
   T xp[2][kDiagP][kNB];          // published rows of inv(L), double-buffered by panel
   T rinv[kNB];                   // 1 / L[j, j]
-  int fail[kNB / kDiagP];        // per panel: failing column or -1
+  int fail[kNB / kDiagP];        // per step: failing column or -1
   int skip;                      // the system has already failed: nothing to do
 };
 
@@ -110,39 +111,41 @@ __device__ __forceinline__ void diag_quad_block(TS* __restrict__ a, int64_t lda,
     }
   }
   __syncthreads();
-  // (no early exit: a failure only turns the remaining steps off, so every loop
-  // keeps its constant trip count and r stays in registers)
-  int fail = -1;
-  static_for(std::make_integer_sequence<int, N / P>{}, [&](auto pi) {
-    constexpr int p = decltype(pi)::value, j0 = p * P, lo = j0 % WC;
-    if (w == p / 2 && fail < 0) {
-      int f = -1;
+  // No branch depends on the data: a failing pivot only records its column in
+  // f and the factor runs on through NaNs, which land in columns >= f, where
+  // the write-back does not look; every loop keeps its constant trip count and
+  // r stays in registers.
+  T rpl = T(0);  // 1 / L[i, i] once this lane's diagonal column is factored
+  // Factors the 8-column panel of this wave's columns starting at lo (block
+  // column j0): pivots and L[k, j] by v_readlane; f becomes the first failing
+  // column.
+  auto factor8 = [&](auto lo_c, int j0, int& f) {
+    constexpr int lo = decltype(lo_c)::value;
 #pragma unroll
-      for (int c = 0; c < P; ++c) {
-        const int j = j0 + c;
-        if (f < 0) {
-          const T d = bcast(r[lo + c], j);
-          if (d <= T(0)) {  // cholesky.rs:69 (NaN passes)
-            f = j;
-          } else {
-            T piv, rp;
-            pivot_rsqrt(d, piv, rp);
-            r[lo + c] = (i == j) ? piv : r[lo + c] * rp;
-            if (i == 0) sm.rinv[j] = rp;
+    for (int c = 0; c < P; ++c) {
+      const int j = j0 + c;
+      const T d = bcast(r[lo + c], j);
+      f = (f < 0 && d <= T(0)) ? j : f;  // cholesky.rs:69 (NaN passes)
+      T piv, rp;
+      pivot_rsqrt(d, piv, rp);
+      r[lo + c] = (i == j) ? piv : r[lo + c] * rp;
+      rpl = (i == j) ? rp : rpl;
 #pragma unroll
-            for (int q = c + 1; q < P; ++q) r[lo + q] -= r[lo + c] * bcast(r[lo + c], j0 + q);
-          }
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < P; q += 2) *reinterpret_cast<v2*>(&l[i * S + j0 + q]) = v2{r[lo + q], r[lo + q + 1]};
-      if (i == 0) sm.fail[p] = f;
+      for (int q = c + 1; q < P; ++q) r[lo + q] -= r[lo + c] * bcast(r[lo + c], j0 + q);
     }
-    __syncthreads();
-    if (fail < 0) fail = sm.fail[p];
-    if (fail < 0 && c0 + WC > j0 + P) {
-      // rank-8 update of this wave's columns right of the panel:
-      // r[k] -= sum_q L[i, j0+q] L[k, j0+q]
+#pragma unroll
+    for (int q = 0; q < P; q += 2) *reinterpret_cast<v2*>(&l[i * S + j0 + q]) = v2{r[lo + q], r[lo + q + 1]};
+  };
+  // Step ws: wave ws factors its two panels (the second after the first's
+  // update of it, from the LDS copy this wave has just written: one wave's LDS
+  // accesses complete in order, so no barrier), then every later wave applies
+  // the rank-16 update to its own columns.
+  static_for(std::make_integer_sequence<int, N / WC>{}, [&](auto wsi) {
+    constexpr int ws = decltype(wsi)::value, j0 = ws * WC;
+    if (w == ws) {
+      int f = -1;
+      factor8(std::integral_constant<int, 0>{}, j0, f);
+      __builtin_amdgcn_wave_barrier();
       T pr[P];
 #pragma unroll
       for (int q = 0; q < P; q += 2) {
@@ -151,20 +154,46 @@ __device__ __forceinline__ void diag_quad_block(TS* __restrict__ a, int64_t lda,
         pr[q + 1] = v[1];
       }
 #pragma unroll
+      for (int kk = P; kk < WC; ++kk) {
+        T t = r[kk];
+#pragma unroll
+        for (int q = 0; q < P; q += 2) {
+          const v2 lk = *reinterpret_cast<const v2*>(&l[(j0 + kk) * S + j0 + q]);
+          t = fma_t(-pr[q], lk[0], fma_t(-pr[q + 1], lk[1], t));
+        }
+        r[kk] = t;
+      }
+      factor8(std::integral_constant<int, P>{}, j0 + P, f);
+      if (i == 0) sm.fail[ws] = f;
+      if (i >= j0 && i < j0 + WC) sm.rinv[i] = rpl;
+    }
+    __syncthreads();
+    if (w > ws) {
+      // rank-16 update of this wave's columns: r[k] -= sum_q L[i, j0+q] L[k, j0+q]
+      T pr[WC];
+#pragma unroll
+      for (int q = 0; q < WC; q += 2) {
+        const v2 v = *reinterpret_cast<const v2*>(&l[i * S + j0 + q]);
+        pr[q] = v[0];
+        pr[q + 1] = v[1];
+      }
+#pragma unroll
       for (int kk = 0; kk < WC; ++kk) {
         const int k = c0 + kk;
-        if (k >= j0 + P) {
-          T t = r[kk];
+        T t = r[kk];
 #pragma unroll
-          for (int q = 0; q < P; q += 2) {
-            const v2 lk = *reinterpret_cast<const v2*>(&l[k * S + j0 + q]);
-            t = fma_t(-pr[q], lk[0], fma_t(-pr[q + 1], lk[1], t));
-          }
-          r[kk] = t;
+        for (int q = 0; q < WC; q += 2) {
+          const v2 lk = *reinterpret_cast<const v2*>(&l[k * S + j0 + q]);
+          t = fma_t(-pr[q], lk[0], fma_t(-pr[q + 1], lk[1], t));
         }
+        r[kk] = t;
       }
     }
   });
+  // the first failing step's column (the steps after it ran on garbage)
+  int fail = -1;
+#pragma unroll
+  for (int ws = 0; ws < N / WC; ++ws) fail = fail < 0 ? sm.fail[ws] : fail;
   // write back from the LDS copy of L (every panel up to the failing one was
   // published there), consecutive lanes on consecutive columns of a row:
   // rows < f (or all rows) with their lower part, row f up to f-1
