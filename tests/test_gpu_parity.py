@@ -136,7 +136,11 @@ def test_invc(n):
 
 # --- error paths ------------------------------------------------------------------
 
-@pytest.mark.parametrize("n,j", [(12, 7), (100, 64), (300, 5), (300, 200), (700, 511)])
+# Blocked orders put the failing column at each position of the 64-column
+# diagonal block's four 16-column steps: 300/5 and 300/200 the first step,
+# 300/213 and 700/600 the second, 300/172 the third, 700/511 the last column.
+@pytest.mark.parametrize("n,j", [(12, 7), (100, 64), (300, 5), (300, 200), (300, 213), (300, 172), (700, 600),
+                                 (700, 511)])
 def test_not_pd_partial_state_matches_reference(n, j):
     rng = np.random.default_rng(n + j)
     a = spd(n, n + j)
@@ -178,6 +182,21 @@ def test_nan_pivot_passes():
     a = np.array([[4.0, 2.0], [2.0, np.nan]])
     l = llh.cholesky(a)
     assert l[0, 0] == 2.0 and l[1, 0] == 1.0 and np.isnan(l[1, 1])
+
+
+@pytest.mark.parametrize("n,j", [(300, 100), (700, 650)])
+def test_nan_pivot_passes_blocked(n, j):
+    # cholesky.rs:69 fails only on d <= 0, so a NaN pivot inside a blocked
+    # diagonal block is not an error: the NaN reaches exactly the entries that
+    # the reference's order reaches (rows and columns >= j of the lower part)
+    a = spd(n, n + j)
+    a[j, j] = np.nan
+    l = np.tril(llh.cholesky_dirty(np.array(a)))
+    ref = np.tril(orc.cholesky_dirty(np.array(a)))
+    assert np.array_equal(np.isnan(l), np.isnan(ref))
+    assert np.isnan(ref[j:, j:][np.tril_indices(n - j)]).all()
+    ok = ~np.isnan(ref)
+    assert np.abs(l[ok] - ref[ok]).max() <= 1e-12 * np.abs(ref[ok]).max()
 
 
 def test_zero_columns_and_empty():
