@@ -502,7 +502,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU layout is described in DESIGN.md, not run).", file=sys.stderr)
+              "(the multi-GPU layout is a design only; nothing here runs on a second GPU).", file=sys.stderr)
         sys.exit(2)
 
     import torch
