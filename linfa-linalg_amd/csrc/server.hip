@@ -2,7 +2,6 @@
 // requests with HBM admission and a resident factor cache.
 #include <algorithm>
 #include <chrono>
-#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -100,12 +99,10 @@ struct Job {
   size_t reserved = 0;
   StepStage* step = nullptr;  // batched jobs share their step's staging and event
   hipError_t error = hipSuccess;  // a failed enqueue: reported instead of reading the staging
-  double cost_us = 0;  // estimated stream time, held in its stream's load until it finishes
 };
 
 struct StreamState {
   hipStream_t s = nullptr;
-  double load_us = 0;  // estimated time of the unfinished jobs enqueued here
   SolveSync sync;
   void* sync_buf = nullptr;
   size_t sync_bytes = 0;
@@ -119,6 +116,7 @@ struct llh_server {
   int device = 0;
   llh_server_config cfg{};
   std::vector<StreamState> streams;
+  int next_stream = 0;
   std::thread worker;
   std::mutex mu;
   std::condition_variable cv_submit, cv_done;
@@ -348,7 +346,6 @@ void finish(llh_server* s, Job& j) {
     }
   }
   StreamState& st = s->streams[j.stream];
-  st.load_us = std::max(0.0, st.load_us - j.cost_us);
   dev_free(st.s, j.dev);
   s->used -= j.reserved;
   s->pinned.put(j.host, j.host_cls);
@@ -367,17 +364,6 @@ void finish(llh_server* s, Job& j) {
     s->stats.completed++;
   }
   s->cv_done.notify_all();
-}
-
-// Estimated stream time of a request: a new factor is a chain of about 50 us
-// per 64 columns plus its flops at ~20 TFLOP/s; each solve pass is two
-// one-launch triangles of about 7 us per 64 rows (FP32 requests run one pass
-// per refinement step more).  Only the ordering of streams uses it.
-double job_cost_us(const llh_request& r, bool factor, llh_precision prec, int ir_iterations) {
-  const double n = (double)r.n, blocks = std::ceil(n / 64.0);
-  double us = factor ? blocks * 50.0 + n * n * n / 3.0 / 2.0e7 : 0.0;
-  const int passes = prec == LLH_PREC_F32 ? 1 + std::max(0, ir_iterations) : 1;
-  return us + passes * 2.0 * blocks * 7.0;
 }
 
 // Admits one pending job: reserves memory, attaches (or creates) its entry.
@@ -400,16 +386,12 @@ bool admit(llh_server* s, Job& j) {
     return true;
   }
   // Small new systems run in the step's batched launch on stream 0, so their
-  // stream-ordered allocations are made there; everything else goes to the
-  // stream with the least estimated unfinished work, so that a short request
-  // does not queue behind a long factor while another stream is idle.
+  // stream-ordered allocations are made there; everything else round-robins.
   if (small_new || s->streams.size() == 1) {
     j.stream = 0;
   } else {
-    j.cost_us = job_cost_us(r, e == nullptr, prec, s->cfg.ir_iterations);
-    j.stream = 1;
-    for (int k = 2; k < (int)s->streams.size(); ++k)
-      if (s->streams[k].load_us < s->streams[j.stream].load_us) j.stream = k;
+    j.stream = 1 + s->next_stream % ((int)s->streams.size() - 1);
+    s->next_stream = (s->next_stream + 1) % ((int)s->streams.size() - 1);
   }
   hipStream_t st = s->streams[j.stream].s;
   size_t hb = (size_t)r.n * r.m * 8 + (2 * r.m + 2) * 8;
@@ -461,7 +443,6 @@ bool admit(llh_server* s, Job& j) {
   j.entry = e;
   j.prec_used = e->prec;
   j.t_admit = now_s();
-  s->streams[j.stream].load_us += j.cost_us;  // released in finish()
   s->peak = std::max(s->peak, s->used);
   return true;
 }
