@@ -78,7 +78,7 @@ def tri_arr(draw, uplo):
     return a
 
 
-@settings(max_examples=300, deadline=None)
+@settings(max_examples=300, deadline=None, derandomize=True)
 @given(hpd_arr())
 def test_cholesky_property(orig):
     """tests/cholesky.rs:21-54 (tolerance 1e-7)."""
@@ -89,7 +89,7 @@ def test_cholesky_property(orig):
     assert np.abs(chol - orc.triangular_inplace(dirty, LOWER)).max() <= 1e-7
 
 
-@settings(max_examples=300, deadline=None)
+@settings(max_examples=300, deadline=None, derandomize=True)
 @given(system_of(hpd_arr()))
 def test_solvec_property(system):
     """tests/cholesky.rs:56-62 (tolerance 1e-5)."""
@@ -98,14 +98,14 @@ def test_solvec_property(system):
     assert np.abs(orc.solvec(np.array(a), b) - x).max() <= 1e-5
 
 
-@settings(max_examples=300, deadline=None)
+@settings(max_examples=300, deadline=None, derandomize=True)
 @given(hpd_arr())
 def test_invc_property(a):
     """tests/cholesky.rs:64-67 (tolerance 1e-7)."""
     assert np.abs(a @ orc.invc(a) - np.eye(a.shape[0])).max() <= 1e-7
 
 
-@settings(max_examples=300, deadline=None)
+@settings(max_examples=300, deadline=None, derandomize=True)
 @given(st.sampled_from([UPPER, LOWER]).flatmap(lambda u: st.tuples(st.just(u),
                                                                     system_of(tri_arr(u)))))
 def test_solve_triangular_property(case):
