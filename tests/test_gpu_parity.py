@@ -163,6 +163,24 @@ def test_not_pd_partial_state_matches_reference(n, j):
     del rng
 
 
+@pytest.mark.parametrize("n,j", [(300, 5), (300, 213), (400, 300), (700, 650)])
+def test_not_pd_partial_state_f32_blocked(n, j):
+    """The f32 blocked path (above the exact kernel's 176) fails at the same
+    column as the reference and leaves its partial state; the rows before the
+    failure agree with the oracle to f32 rounding."""
+    a = spd(n, n + j, dtype=np.float32)
+    a[j, j] = -1.0
+    work, ref = np.array(a), np.array(a)
+    with pytest.raises(llh.NotPositiveDefinite) as ei:
+        llh.cholesky_inplace_dirty(work)
+    assert ei.value.column == j
+    with pytest.raises(orc.OracleError):
+        orc.cholesky_inplace_dirty(ref)
+    assert np.abs(work[:j + 1] - ref[:j + 1]).max() <= 1e-4
+    assert work[j, j] == a[j, j]
+    assert np.array_equal(work[j + 1:], a[j + 1:])
+
+
 def test_error_precedence():
     with pytest.raises(llh.NotSquare) as ei:
         llh.solvec(np.zeros((2, 3)), np.zeros((2, 1)))
