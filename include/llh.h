@@ -103,6 +103,14 @@ llh_status llh_context_set_profiling_classes(llh_context* ctx, uint32_t class_ma
 llh_status llh_context_kernel_stats(llh_context* ctx, int kernel_class, llh_kernel_stats* out,
                                     llh_error* err);
 
+/* Bound on each flag wait of the one-launch triangular solve (no counterpart in
+ * the reference; for tests of the error path).  Past `limit` polls a waiting
+ * workgroup gives up and the call reports LLH_ERR_HIP ("a block wait exceeded
+ * its bound") instead of hanging; limit < 0 makes every wait give up at its
+ * first poll, so a solve of more than one 64-row block fails at once while every
+ * kernel still exits normally.  0 restores the default (2^22 polls). */
+llh_status llh_context_set_solve_spin_limit(llh_context* ctx, int64_t limit, llh_error* err);
+
 /* ---------------------------------------------------------------------------
  * Plain forms: host memory, any strides.  _d = f64, _s = f32.
  * ------------------------------------------------------------------------- */
@@ -326,8 +334,11 @@ void llh_batch_plan_destroy(llh_batch_plan* plan);
  * Request buffers are the caller's and must stay valid until the request is
  * done: a (n x n, f64, lower triangle read) in host memory, or already on the
  * device when a_on_device != 0; b (n x m) and x (n x m) in host memory,
- * row-major.  LLH_PREC_F32 requests factor in f32 (f64 diagonal blocks) and
- * apply `ir_iterations` f64 refinement steps; f64 requests solve directly.
+ * row-major.  The server reads `a` only while that request runs: an
+ * LLH_PREC_F32 entry keeps its own f64 copy of A for the refinement of later
+ * requests that reuse it.  LLH_PREC_F32 requests factor in f32 (f64 diagonal
+ * blocks) and apply `ir_iterations` f64 refinement steps; f64 requests solve
+ * directly.
  * ------------------------------------------------------------------------- */
 typedef struct llh_server llh_server;
 
@@ -374,6 +385,22 @@ llh_status llh_server_submit(llh_server* srv, const llh_request* req, uint64_t* 
 int llh_server_poll(llh_server* srv, uint64_t ticket, llh_result* out);
 llh_status llh_server_wait(llh_server* srv, uint64_t ticket, llh_result* out, llh_error* err);
 void llh_server_get_stats(llh_server* srv, llh_server_stats* out);
+/* Test hooks (no counterpart in the reference; for tests of the error paths):
+ * failures forced from the host, so every kernel still exits normally.
+ *   LLH_SERVER_INJECT_STAGING: the next `value` batched steps fail to stage their
+ *     descriptors; every request of such a step finishes with LLH_ERR_HIP.
+ *   LLH_SERVER_INJECT_EVENT: the value-th event the server takes from now on
+ *     fails to be created (1: the next one); the request that needed it
+ *     finishes with LLH_ERR_HIP and the server goes on.
+ *   LLH_SERVER_INJECT_SOLVE_SPIN: the flag-wait bound of the server's one-launch
+ *     solves, as in llh_context_set_solve_spin_limit (0: the default); a request
+ *     whose sweep gives up finishes with LLH_ERR_HIP. */
+enum {
+  LLH_SERVER_INJECT_STAGING = 1,
+  LLH_SERVER_INJECT_EVENT = 2,
+  LLH_SERVER_INJECT_SOLVE_SPIN = 3
+};
+llh_status llh_server_inject(llh_server* srv, int kind, int64_t value, llh_error* err);
 /* Drops resident factors that no request is using. */
 void llh_server_evict_all(llh_server* srv);
 /* Finishes in-flight work, then frees everything. */

@@ -128,8 +128,10 @@ hipError_t ctx_sync(llh_context* ctx, int n, int m, SolveSync* sync) {
   }
   sync->words = static_cast<int32_t*>(ctx->sync.ptr);
   sync->nflags = (int64_t)(ctx->sync.bytes / sizeof(int32_t)) - 2;
-  sync->epoch = ctx->epoch;
-  return hipSuccess;
+  sync->epoch = &ctx->epoch;
+  sync->spin_limit = ctx->spin_limit;
+  // one error word per call: cleared here, read back once the call's sweeps are done
+  return solve_sync_begin(*sync, ctx->stream);
 }
 
 hipError_t ctx_read_solve_error(llh_context* ctx) {
@@ -148,9 +150,6 @@ namespace {
 llh_status solve_stalled(llh_error* err) {
   return set_error(err, LLH_ERR_HIP, 0, 0, -1, "triangular solve: a block wait exceeded its bound", 0);
 }
-
-// Runs after every persistent solve: records the advanced epoch.
-void commit_sync(llh_context* ctx, const SolveSync& s) { ctx->epoch = s.epoch; }
 
 #define CHECK_HIP(expr, where)                           \
   do {                                                   \
@@ -335,7 +334,6 @@ llh_status solve_triangular_plain(llh_context* ctx, size_t rows, size_t cols, co
                             (int64_t)b_cols, nullptr, false, static_cast<T*>(dinv), &sync, ctx->stream,
                             ctx->profiler()),
             "trsm");
-  commit_sync(ctx, sync);
   CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   CHECK_HIP(hipMemcpyAsync(hb.data(), db, n * b_cols * sizeof(T), hipMemcpyDeviceToHost, ctx->stream),
             "download");
@@ -366,12 +364,11 @@ llh_status solve_on_device(llh_context* ctx, size_t n, T* da, size_t m, T* db, l
                                                false, ctx_lookahead(ctx)),
               "potrf");
     CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, dinv, false, nullptr,
-                              &sync, ctx->stream, ctx->profiler()),
+                              &sync, ctx->stream, ctx->profiler(), ctx->d_info),
               "forward solve");
     CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, dinv, true, nullptr,
-                              &sync, ctx->stream, ctx->profiler()),
+                              &sync, ctx->stream, ctx->profiler(), ctx->d_info),
               "back solve");
-    commit_sync(ctx, sync);
   }
   CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   return LLH_OK;
@@ -473,7 +470,6 @@ llh_status dev_solve_triangular(llh_context* ctx, llh_uplo uplo, int trans, size
   CHECK_HIP(trsm_enqueue<T>(uplo == LLH_UPPER, trans != 0, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb,
                             nullptr, false, static_cast<T*>(dinv), &sync, ctx->stream, ctx->profiler()),
             "trsm");
-  commit_sync(ctx, sync);
   CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
   if (ctx->h_info[2] != 0) return solve_stalled(err);
@@ -497,15 +493,16 @@ llh_status dev_solvec(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T*
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
             "status");
   if (m > 0) {
-    // Solves run regardless of the status and are discarded on failure; this
-    // keeps one synchronisation per call.
+    // The solves are enqueued at once (one synchronisation per call) but gated
+    // on the factor's status word: after NotPositiveDefinite every solve kernel
+    // is a no-op and B keeps the caller's values, as the reference returns
+    // before touching b (cholesky.rs:140).
     CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, false,
-                              nullptr, &sync, ctx->stream, ctx->profiler()),
+                              nullptr, &sync, ctx->stream, ctx->profiler(), ctx->d_info),
               "forward solve");
     CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, true, nullptr,
-                              &sync, ctx->stream, ctx->profiler()),
+                              &sync, ctx->stream, ctx->profiler(), ctx->d_info),
               "back solve");
-    commit_sync(ctx, sync);
   }
   CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
@@ -564,7 +561,6 @@ llh_status dev_orthonormalize(llh_context* ctx, size_t n, size_t k, T* v, size_t
   CHECK_HIP(trsm_enqueue<T>(false, false, K, r, (int64_t)ldr, K, w, (int64_t)K, dinv, false, nullptr, &sync, s,
                             ctx->profiler()),
             "inverse");
-  commit_sync(ctx, sync);
   if (n > 0) {
     if (K <= 64) {
       // one 64-column tile holds all k columns of its rows: in place
@@ -609,7 +605,6 @@ llh_status dev_apply_constraints(llh_context* ctx, size_t n, size_t k, T* v, siz
   CHECK_HIP(trsm_enqueue<T>(false, false, KY, l_yy, (int64_t)ldl, K, g, (int64_t)K, nullptr, false,
                             static_cast<T*>(dv), &sync, s, ctx->profiler()),
             "solve");
-  commit_sync(ctx, sync);
   CHECK_HIP(launch_gemm<T>(false, false, false, N, K, KY, T(-1), y, (int64_t)ldy, g, (int64_t)K, T(1), v,
                            (int64_t)ldv, nullptr, s),
             "update");
@@ -692,6 +687,12 @@ llh_status llh_context_set_stream(llh_context* ctx, void* hip_stream, llh_error*
 }
 
 void* llh_context_stream(llh_context* ctx) { return bad_ctx(ctx) ? nullptr : (void*)ctx->stream; }
+
+llh_status llh_context_set_solve_spin_limit(llh_context* ctx, int64_t limit, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  ctx->spin_limit = limit == 0 ? kSpinLimit : limit;
+  return ok(err);
+}
 
 llh_status llh_context_set_profiling(llh_context* ctx, int enable, llh_error* err) {
   if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");

@@ -187,24 +187,25 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
 template <typename T>
 hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t lda, int m, T* b, int64_t ldb,
                         const T* dinv, bool dinv_t, T* dinv_work, SolveSync* sync, hipStream_t s,
-                        Profiler* prof) {
+                        Profiler* prof, const int32_t* gate) {
   if (n <= 0 || m <= 0) return hipSuccess;
   const double es = (double)sizeof(T);
   const int nblk0 = (n + kNB - 1) / kNB;
   const int nq = (m + kSolveChunkHost - 1) / kSolveChunkHost;
-  if (sync != nullptr && m <= 64 && sync->nflags >= (int64_t)nblk0 * nq && (dinv != nullptr || dinv_work)) {
+  if (sync != nullptr && sync->epoch != nullptr && m <= 64 && sync->nflags >= (int64_t)nblk0 * nq &&
+      (dinv != nullptr || dinv_work)) {
     if (dinv == nullptr) {
       LLH_PROF(kSolveDiag, (double)n * kNB * kNB / 3.0, es * (double)n * kNB,
-               (launch_trtri_blocks<T>(n, a, lda, trans, upper_op, dinv_work, nullptr, s)));
+               (launch_trtri_blocks<T>(n, a, lda, trans, upper_op, dinv_work, gate, s)));
       dinv = dinv_work;
       dinv_t = false;
     }
-    LLH_TRY(launch_fill_i32(sync->words, 0, 2, s));
+    // the launcher resets the ticket; the error word is the call's (solve_sync_begin)
+    const int32_t epoch = (*sync->epoch)++;
     const double fl = (double)n * n * m;
     LLH_PROF(kSolveGemm, fl, es * ((double)n * n / 2.0 + 2.0 * n * m),
              (launch_trsm_persistent<T>(n, m, a, lda, trans, !upper_op, dinv, dinv_t, b, ldb, sync->words,
-                                        sync->words + 2, sync->epoch, sync->words + 1, kSpinLimit, s)));
-    sync->epoch += 1;
+                                        sync->words + 2, epoch, sync->words + 1, sync->spin_limit, gate, s)));
     return hipSuccess;
   }
   const int nblk = (n + kNB - 1) / kNB;
@@ -213,27 +214,27 @@ hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t ld
       const int i0 = bi * kNB, bs = std::min(kNB, n - i0);
       const T* tkk = a + (int64_t)i0 * lda + i0;
       LLH_PROF(kSolveDiag, (double)bs * bs * m, es * ((double)bs * bs / 2.0 + 2.0 * bs * m),
-               (launch_trsv_block<T>(false, trans, bs, tkk, lda, m, b + (int64_t)i0 * ldb, ldb, s)));
+               (launch_trsv_block<T>(false, trans, bs, tkk, lda, m, b + (int64_t)i0 * ldb, ldb, gate, s)));
       const int rest = n - i0 - bs;
       if (rest <= 0) continue;
       // B[i0+bs:, :] -= op(A)[i0+bs:, i0:i0+bs] * X[i0:i0+bs, :]
       const T* blk = trans ? a + (int64_t)i0 * lda + (i0 + bs) : a + (int64_t)(i0 + bs) * lda + i0;
       LLH_PROF(kSolveGemm, 2.0 * rest * m * bs, es * ((double)rest * bs + 2.0 * rest * m + (double)bs * m),
                (launch_gemm<T>(trans, false, false, rest, m, bs, T(-1), blk, lda, b + (int64_t)i0 * ldb, ldb,
-                               T(1), b + (int64_t)(i0 + bs) * ldb, ldb, nullptr, s)));
+                               T(1), b + (int64_t)(i0 + bs) * ldb, ldb, gate, s)));
     }
   } else {  // backward
     for (int bi = nblk - 1; bi >= 0; --bi) {
       const int i0 = bi * kNB, bs = std::min(kNB, n - i0);
       const T* tkk = a + (int64_t)i0 * lda + i0;
       LLH_PROF(kSolveDiag, (double)bs * bs * m, es * ((double)bs * bs / 2.0 + 2.0 * bs * m),
-               (launch_trsv_block<T>(true, trans, bs, tkk, lda, m, b + (int64_t)i0 * ldb, ldb, s)));
+               (launch_trsv_block<T>(true, trans, bs, tkk, lda, m, b + (int64_t)i0 * ldb, ldb, gate, s)));
       if (i0 == 0) continue;
       // B[:i0, :] -= op(A)[:i0, i0:i0+bs] * X[i0:i0+bs, :]
       const T* blk = trans ? a + (int64_t)i0 * lda : a + i0;
       LLH_PROF(kSolveGemm, 2.0 * i0 * m * bs, es * ((double)i0 * bs + 2.0 * i0 * m + (double)bs * m),
                (launch_gemm<T>(trans, false, false, i0, m, bs, T(-1), blk, lda, b + (int64_t)i0 * ldb, ldb,
-                               T(1), b, ldb, nullptr, s)));
+                               T(1), b, ldb, gate, s)));
     }
   }
   return hipSuccess;
@@ -244,8 +245,15 @@ template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool,
 template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, float*, hipStream_t, Profiler*,
                                          bool, const Lookahead*, _Float16*);
 template hipError_t trsm_enqueue<double>(bool, bool, int, const double*, int64_t, int, double*, int64_t,
-                                         const double*, bool, double*, SolveSync*, hipStream_t, Profiler*);
+                                         const double*, bool, double*, SolveSync*, hipStream_t, Profiler*,
+                                         const int32_t*);
 template hipError_t trsm_enqueue<float>(bool, bool, int, const float*, int64_t, int, float*, int64_t,
-                                        const float*, bool, float*, SolveSync*, hipStream_t, Profiler*);
+                                        const float*, bool, float*, SolveSync*, hipStream_t, Profiler*,
+                                        const int32_t*);
+
+hipError_t solve_sync_begin(const SolveSync& sync, hipStream_t s) {
+  if (sync.words == nullptr) return hipSuccess;
+  return launch_fill_i32(sync.words + 1, 0, 1, s);
+}
 
 }  // namespace llh

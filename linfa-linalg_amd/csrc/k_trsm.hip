@@ -66,7 +66,9 @@ __global__ __launch_bounds__(64) void trsm_rows_kernel(int m, int b, T* __restri
 
 template <typename T, bool UPPER_OP>
 __global__ __launch_bounds__(64) void trsv_block_kernel(int b, bool trans, const T* __restrict__ Tm,
-                                                        int64_t ldt, int m, T* __restrict__ Y, int64_t ldy) {
+                                                        int64_t ldt, int m, T* __restrict__ Y, int64_t ldy,
+                                                        const int32_t* __restrict__ gate) {
+  if (gate != nullptr && *gate != kNoFail) return;
   // St[c * kNB + r] = op(T)[r, c], only op(T)'s triangle: column c of op(T) is
   // contiguous, which is what the column sweep reads.
   __shared__ __attribute__((aligned(16))) T St[kNB * kNB];
@@ -152,6 +154,10 @@ __global__ __launch_bounds__(256) void trtri64_kernel(int n, const TS* __restric
 // and the rest of the block's work overlaps the two steps before it.
 // The wait is bounded: past spin_limit polls (or once another workgroup has
 // given up) the workgroup records an error and returns without raising its flag.
+// A negative spin_limit makes every wait give up at once (a host-forced failure
+// for tests of the error path; every workgroup still exits normally).  With a
+// gate (the factor's status word) that holds a failed row, every workgroup
+// returns at once and B is left untouched.
 constexpr int kSolveChunk = kSolveChunkHost;
 constexpr int kFlagWindow = 64;  // flags checked ahead per wait
 
@@ -161,7 +167,8 @@ template <typename T, bool VEC>
 __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void trsm_persistent_kernel(
     int n, int m, const T* __restrict__ A, int64_t lda, bool trans, bool lower_op, const T* __restrict__ Dinv,
     bool dinv_t, T* B, int64_t ldb, int32_t* ticket, int32_t* flags, int32_t epoch, int32_t* err,
-    int64_t spin_limit) {
+    int64_t spin_limit, const int32_t* __restrict__ gate) {
+  if (gate != nullptr && *gate != kNoFail) return;
   constexpr int MC = kSolveChunk, KW = kNB / 4, S = kNB + 1, SL = kNB * MC;
   static_assert(MC == 16 && KW == 16, "one 16-column chunk, 16 tile columns per wave");
   static_assert(5 * SL >= kNB * S, "a staged tile fits the work area");
@@ -247,7 +254,11 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void trsm_persistent_k
     if (tid == 0) {
       int64_t it = 0;
       const int32_t* f = &flags[jof(st) * nq + q];
-      while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+      if (spin_limit < 0) {
+        atomicExch(err, 1);
+        s_abort = 1;
+      }
+      while (!s_abort && __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
         if (++it > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
           atomicExch(err, 1);
           s_abort = 1;
@@ -413,14 +424,14 @@ hipError_t launch_trsm_rows(int m, int b, T* X, int64_t ldx, const T* L, int64_t
 
 template <typename T>
 hipError_t launch_trsv_block(bool upper_op, bool trans, int b, const T* Tm, int64_t ldt, int m, T* Y,
-                             int64_t ldy, hipStream_t s) {
+                             int64_t ldy, const int32_t* gate, hipStream_t s) {
   if (m <= 0 || b <= 0) return hipSuccess;
   if (b > kNB) return hipErrorInvalidValue;
   const dim3 grid((m + 63) / 64);
   if (upper_op)
-    hipLaunchKernelGGL((trsv_block_kernel<T, true>), grid, dim3(64), 0, s, b, trans, Tm, ldt, m, Y, ldy);
+    hipLaunchKernelGGL((trsv_block_kernel<T, true>), grid, dim3(64), 0, s, b, trans, Tm, ldt, m, Y, ldy, gate);
   else
-    hipLaunchKernelGGL((trsv_block_kernel<T, false>), grid, dim3(64), 0, s, b, trans, Tm, ldt, m, Y, ldy);
+    hipLaunchKernelGGL((trsv_block_kernel<T, false>), grid, dim3(64), 0, s, b, trans, Tm, ldt, m, Y, ldy, gate);
   return hipGetLastError();
 }
 
@@ -437,7 +448,7 @@ template <typename T>
 hipError_t launch_trsm_persistent(int n, int m, const T* A, int64_t lda, bool trans, bool lower_op,
                                   const T* Dinv, bool dinv_t, T* B, int64_t ldb, int32_t* ticket,
                                   int32_t* flags, int32_t epoch, int32_t* err, int64_t spin_limit,
-                                  hipStream_t s) {
+                                  const int32_t* gate, hipStream_t s) {
   if (n <= 0 || m <= 0) return hipSuccess;
   const int nb = (n + kNB - 1) / kNB, nq = (m + kSolveChunk - 1) / kSolveChunk;
   hipError_t e = launch_fill_i32(ticket, 0, 1, s);
@@ -445,10 +456,10 @@ hipError_t launch_trsm_persistent(int n, int m, const T* A, int64_t lda, bool tr
   const bool vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && ((lda * sizeof(T)) % 16 == 0);
   if (vec)
     hipLaunchKernelGGL((trsm_persistent_kernel<T, true>), dim3(nb * nq), dim3(256), 0, s, n, m, A, lda, trans,
-                       lower_op, Dinv, dinv_t, B, ldb, ticket, flags, epoch, err, spin_limit);
+                       lower_op, Dinv, dinv_t, B, ldb, ticket, flags, epoch, err, spin_limit, gate);
   else
     hipLaunchKernelGGL((trsm_persistent_kernel<T, false>), dim3(nb * nq), dim3(256), 0, s, n, m, A, lda, trans,
-                       lower_op, Dinv, dinv_t, B, ldb, ticket, flags, epoch, err, spin_limit);
+                       lower_op, Dinv, dinv_t, B, ldb, ticket, flags, epoch, err, spin_limit, gate);
   return hipGetLastError();
 }
 
@@ -460,17 +471,17 @@ template hipError_t launch_trtri_blocks<float, double>(int, const float*, int64_
                                                        const int32_t*, hipStream_t);
 template hipError_t launch_trsm_persistent<double>(int, int, const double*, int64_t, bool, bool, const double*,
                                                    bool, double*, int64_t, int32_t*, int32_t*, int32_t, int32_t*,
-                                                   int64_t, hipStream_t);
+                                                   int64_t, const int32_t*, hipStream_t);
 template hipError_t launch_trsm_persistent<float>(int, int, const float*, int64_t, bool, bool, const float*, bool,
                                                   float*, int64_t, int32_t*, int32_t*, int32_t, int32_t*, int64_t,
-                                                  hipStream_t);
+                                                  const int32_t*, hipStream_t);
 template hipError_t launch_trsm_rows<double>(int, int, double*, int64_t, const double*, int64_t,
                                              const int32_t*, hipStream_t);
 template hipError_t launch_trsm_rows<float>(int, int, float*, int64_t, const float*, int64_t, const int32_t*,
                                             hipStream_t);
 template hipError_t launch_trsv_block<double>(bool, bool, int, const double*, int64_t, int, double*, int64_t,
-                                              hipStream_t);
+                                              const int32_t*, hipStream_t);
 template hipError_t launch_trsv_block<float>(bool, bool, int, const float*, int64_t, int, float*, int64_t,
-                                             hipStream_t);
+                                             const int32_t*, hipStream_t);
 
 }  // namespace llh

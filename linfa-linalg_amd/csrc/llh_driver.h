@@ -7,13 +7,22 @@
 namespace llh {
 
 // Device words of the one-launch triangular solve: ticket counter, error word
-// and one flag per (64-row block, 16-column chunk); epoch is a value the flags
-// have never held (the context increments it per launch).
+// and one flag per (64-row block, 16-column chunk).  *epoch is the owner's
+// counter (a context's or a server stream's): the next value the flags have
+// never held; each launch takes it and advances it at once, so no later launch
+// can reuse an epoch whatever path the caller takes afterwards.  The error word
+// is cleared once per call (solve_sync_begin), not per sweep, so a sweep that
+// gave up stays visible after the sweeps that follow it.
 struct SolveSync {
   int32_t* words = nullptr;  // [0] ticket, [1] error, [2..] flags
   int64_t nflags = 0;
-  int32_t epoch = 0;
+  int32_t* epoch = nullptr;
+  int64_t spin_limit = kSpinLimit;  // bound on each flag wait (negative: give up at once)
 };
+
+// Enqueues the clearing of the solve error word: once per call, before its
+// first sweep.
+hipError_t solve_sync_begin(const SolveSync& sync, hipStream_t s);
 
 // Number of elements of the diagonal-block inverse buffer for order n.
 inline int64_t dinv_elems(int n) { return (int64_t)((n + kNB - 1) / kNB) * kNB * kNB; }
@@ -48,9 +57,11 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
 // inverses of op(A)'s 64 x 64 diagonal blocks (transposed when dinv_t);
 // otherwise they are computed into dinv_work.  With sync and m <= 64 the solve
 // is one persistent launch; otherwise it is a sequence of blocked launches.
+// gate (optional): the factor's status word; once it holds a failed row every
+// kernel of the solve is a no-op and B keeps its values.
 template <typename T>
 hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t lda, int m, T* b, int64_t ldb,
                         const T* dinv, bool dinv_t, T* dinv_work, SolveSync* sync, hipStream_t s,
-                        Profiler* prof = nullptr);
+                        Profiler* prof = nullptr, const int32_t* gate = nullptr);
 
 }  // namespace llh

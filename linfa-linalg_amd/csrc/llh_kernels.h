@@ -167,7 +167,7 @@ hipError_t launch_trsm_rows(int m, int b, T* X, int64_t ldx, const T* L, int64_t
 // of a row-major Y; upper_op selects back substitution; trans reads T^T.
 template <typename T>
 hipError_t launch_trsv_block(bool upper_op, bool trans, int b, const T* Tm, int64_t ldt, int m, T* Y,
-                             int64_t ldy, hipStream_t s);
+                             int64_t ldy, const int32_t* gate, hipStream_t s);
 
 // k_trsm.hip: inverses of the 64 x 64 diagonal blocks of op(A) (n x n), written
 // as consecutive row-major 64 x 64 blocks at out; upper names op(A)'s triangle.
@@ -177,12 +177,13 @@ hipError_t launch_trtri_blocks(int n, const TS* A, int64_t lda, bool trans, bool
 // k_trsm.hip: one-launch blocked solve op(A) X = B (B row-major n x m) using the
 // diagonal-block inverses Dinv (transposed when dinv_t).  ticket/flags/err are
 // device words (flags: one per 64-row block and 16-column chunk), epoch a value
-// the flags have never held.  *err becomes non-zero if a wait exceeded spin_limit.
+// the flags have never held.  *err becomes non-zero if a wait exceeded spin_limit
+// (a negative spin_limit: at the first wait).
 template <typename T>
 hipError_t launch_trsm_persistent(int n, int m, const T* A, int64_t lda, bool trans, bool lower_op,
                                   const T* Dinv, bool dinv_t, T* B, int64_t ldb, int32_t* ticket,
                                   int32_t* flags, int32_t epoch, int32_t* err, int64_t spin_limit,
-                                  hipStream_t s);
+                                  const int32_t* gate, hipStream_t s);
 
 // k_mixed.hip: conversions, symmetric residual and norms of the refinement.
 // Y = diag(rs) X diag(cs) in f32 (null scales are 1).

@@ -136,6 +136,10 @@ llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error
   return not_pd(err, ctx->h_info[0]);
 }
 
+llh_status solve_stalled_mixed(llh_error* err) {
+  return set_error(err, LLH_ERR_HIP, 0, 0, -1, "triangular solve: a block wait exceeded its bound", 0);
+}
+
 template <typename T>
 hipError_t solve_low(llh_context* ctx, const llh_factor* f, int m, T* d, SolveSync* sync) {
   const int n = f->n;
@@ -178,7 +182,10 @@ llh_status solve_core(llh_context* ctx, llh_factor* f, size_t m_, const double* 
     CHECK_HIP(launch_colmax(n, m, r, m, cm, s, rs, false), "residual norm");
     CHECK_HIP(launch_colmax(n, m, x, (int64_t)ldx, cm + m, s, rs, true), "solution norm");
     CHECK_HIP(hipMemcpyAsync(hn, cm, sizeof(double) * 2 * m, hipMemcpyDeviceToHost, s), "norm readback");
+    CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
     CHECK_HIP(hipStreamSynchronize(s), "synchronize");
+    // a sweep that gave up left x unfinished: report it before any test of x
+    if (ctx->h_info[2] != 0) return solve_stalled_mixed(err);
     bool c = true;
     double worst = 0.0;
     for (int j = 0; j < m; ++j) {
@@ -231,11 +238,9 @@ llh_status solve_core(llh_context* ctx, llh_factor* f, size_t m_, const double* 
       inf.iterations = it + 1;
     }
   }
-  ctx->epoch = sync.epoch;
   CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   CHECK_HIP(hipStreamSynchronize(s), "synchronize");
-  if (ctx->h_info[2] != 0)
-    return set_error(err, LLH_ERR_HIP, 0, 0, -1, "triangular solve: a block wait exceeded its bound", 0);
+  if (ctx->h_info[2] != 0) return solve_stalled_mixed(err);
   if (info) *info = inf;
   if (!inf.converged) return set_error(err, LLH_ERR_NO_CONVERGENCE, 0, 0, -1, "refinement did not converge", 0);
   return ok(err);

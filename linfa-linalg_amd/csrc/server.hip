@@ -1,6 +1,7 @@
 // Solve server (include/llh.h: llh_server_*): continuous batching of SPD solve
 // requests with HBM admission and a resident factor cache.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -27,16 +28,16 @@ double now_s() {
 
 size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// A resident factor: L (f64 or f32), its diagonal-block inverses, and the f64 A
-// the f32 refinement needs.
+// A resident factor: L (f64 or f32), its diagonal-block inverses, and (f32
+// entries) the server's own f64 copy of A that every refinement of a later
+// request reads: a caller's buffer only has to outlive its own request.
 struct Entry {
   uint64_t id = 0;
   int n = 0;
   llh_precision prec = LLH_PREC_F64;
   void* l = nullptr;
   void* dinv = nullptr;
-  const double* a = nullptr;  // f64 A: server copy (own_a) or the caller's device matrix
-  double* a_own = nullptr;
+  double* a_own = nullptr;    // f32 entries: f64 copy of A
   int32_t* d_info = nullptr;  // factor status word; ||A||_inf (f64) at byte 8
   bool orphan = false;        // replaced in the cache; freed when its last job ends
   size_t bytes = 0;
@@ -45,6 +46,7 @@ struct Entry {
   uint64_t last_use = 0;
   bool failed = false;
   int64_t failed_row = -1;
+  hipError_t error = hipSuccess;  // an enqueue of its factor failed
   bool dinv_ready = false;
   bool in_step = false;  // factored by the batched step that has not been launched yet
 };
@@ -99,11 +101,18 @@ struct Job {
   size_t reserved = 0;
   StepStage* step = nullptr;  // batched jobs share their step's staging and event
   hipError_t error = hipSuccess;  // a failed enqueue: reported instead of reading the staging
+  bool settled = false;           // no event: its stream was synchronized, its work is complete
 };
+
+// Host staging of a job (pinned): x (n m doubles) | residual and solution
+// norms (2 m doubles) | factor status (int32) and solve error word (int32) |
+// ||A||_inf (double).
+inline int32_t* stage_info(double* hx, size_t nm, int m) { return reinterpret_cast<int32_t*>(hx + nm + 2 * m); }
 
 struct StreamState {
   hipStream_t s = nullptr;
   SolveSync sync;
+  int32_t epoch = 1;  // the flags' epoch counter of this stream's solves
   void* sync_buf = nullptr;
   size_t sync_bytes = 0;
 };
@@ -135,19 +144,31 @@ struct llh_server {
   llh_server_stats stats{};
   llh_error last_error{};
   bool evict_requested = false;
+  // test hooks (llh_server_inject): failures to force from the host
+  std::atomic<int> inject_staging{0}, inject_event{0};
+  std::atomic<int64_t> spin_limit{kSpinLimit};
 };
 
 namespace {
 
-hipEvent_t get_event(llh_server* s) {
+// An event from the pool or a new one; *out is null when creation failed.
+hipError_t get_event(llh_server* s, hipEvent_t* out) {
+  *out = nullptr;
+  const int countdown = s->inject_event.load();
+  if (countdown > 0) {  // test hook: the countdown-th request from now fails
+    s->inject_event.store(countdown - 1);
+    if (countdown == 1) return hipErrorOutOfMemory;
+  }
   if (!s->event_pool.empty()) {
-    hipEvent_t e = s->event_pool.back();
+    *out = s->event_pool.back();
     s->event_pool.pop_back();
-    return e;
+    return hipSuccess;
   }
   hipEvent_t e = nullptr;
-  (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
-  return e;
+  hipError_t x = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+  if (x != hipSuccess) return x;
+  *out = e;
+  return hipSuccess;
 }
 
 size_t es_of(llh_precision p) { return p == LLH_PREC_F32 ? 4 : 8; }
@@ -155,7 +176,7 @@ size_t es_of(llh_precision p) { return p == LLH_PREC_F32 ? 4 : 8; }
 size_t entry_bytes(const llh_request& r, llh_precision prec) {
   const size_t n = (size_t)r.n;
   size_t b = n * n * es_of(prec) + (size_t)dinv_elems(r.n) * es_of(prec) + 256;
-  if (!r.a_on_device) b += n * n * 8;
+  if (prec == LLH_PREC_F32) b += n * n * 8;  // the entry's own f64 A
   return b;
 }
 
@@ -191,9 +212,11 @@ bool make_room(llh_server* s, size_t need) {
   return true;
 }
 
-hipError_t ensure_sync(StreamState& st, int n, int m) {
+hipError_t ensure_sync(llh_server* s, StreamState& st, int n, int m) {
   const size_t nb = (n + kNB - 1) / kNB, nq = (m + kSolveChunkHost - 1) / kSolveChunkHost;
   const size_t want = (2 + nb * nq) * sizeof(int32_t);
+  st.sync.epoch = &st.epoch;
+  st.sync.spin_limit = s->spin_limit.load();
   if (st.sync_bytes >= want) return hipSuccess;
   hipError_t e = hipStreamSynchronize(st.s);
   if (e != hipSuccess) return e;
@@ -206,7 +229,7 @@ hipError_t ensure_sync(StreamState& st, int n, int m) {
     return e;
   st.sync.words = static_cast<int32_t*>(st.sync_buf);
   st.sync.nflags = (int64_t)(bytes / 4) - 2;
-  st.sync.epoch = 1;
+  st.epoch = 1;
   return hipSuccess;
 }
 
@@ -219,23 +242,25 @@ hipError_t solve_with(StreamState& st, const Entry* e, int m, T* d) {
   return trsm_enqueue<T>(true, true, e->n, l, e->n, m, d, m, dinv, true, nullptr, &st.sync, st.s);
 }
 
-// Enqueues the factor of a new entry on stream st (scratch: n doubles).
+// Enqueues the factor of a new entry on stream st (scratch: n doubles).  The
+// request's A (host or device) is read only here, on st, before the request
+// completes.
 hipError_t enqueue_factor(StreamState& st, Entry* e, const llh_request& r, double* scratch) {
   const int n = e->n;
+  const size_t bytes = (size_t)n * n * 8;
+  const hipMemcpyKind kind = r.a_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   hipError_t x;
-  if (!r.a_on_device) {
-    if ((x = hipMemcpyAsync(e->a_own, r.a, (size_t)n * n * 8, hipMemcpyHostToDevice, st.s)) != hipSuccess) return x;
-  }
   if (e->prec == LLH_PREC_F32) {
+    if ((x = hipMemcpyAsync(e->a_own, r.a, bytes, kind, st.s)) != hipSuccess) return x;
     float* l = static_cast<float*>(e->l);
     double* anorm = reinterpret_cast<double*>(reinterpret_cast<char*>(e->d_info) + 8);
-    if ((x = launch_symm_abs_rowsum(n, e->a, n, scratch, st.s)) != hipSuccess) return x;
+    if ((x = launch_symm_abs_rowsum(n, e->a_own, n, scratch, st.s)) != hipSuccess) return x;
     if ((x = launch_colmax(n, 1, scratch, 1, anorm, st.s)) != hipSuccess) return x;
-    if ((x = launch_convert_d2s(n, n, e->a, n, l, n, true, st.s)) != hipSuccess) return x;
+    if ((x = launch_convert_d2s(n, n, e->a_own, n, l, n, true, st.s)) != hipSuccess) return x;
     x = potrf_enqueue<float>(l, n, n, e->d_info, false, static_cast<float*>(e->dinv), st.s, nullptr, true);
   } else {
     double* l = static_cast<double*>(e->l);
-    if ((x = hipMemcpyAsync(l, e->a, (size_t)n * n * 8, hipMemcpyDeviceToDevice, st.s)) != hipSuccess) return x;
+    if ((x = hipMemcpyAsync(l, r.a, bytes, kind, st.s)) != hipSuccess) return x;
     x = potrf_enqueue<double>(l, n, n, e->d_info, false, static_cast<double*>(e->dinv), st.s);
   }
   if (x != hipSuccess) return x;
@@ -258,7 +283,9 @@ hipError_t enqueue_solve(llh_server* s, Job& j) {
   double* dn = reinterpret_cast<double*>(base + 3 * round_up(nm * 8, 256) + round_up(nm * 4, 256));
   double* hx = static_cast<double*>(j.host);
   hipError_t x;
-  if ((x = ensure_sync(st, n, m)) != hipSuccess) return x;
+  if ((x = ensure_sync(s, st, n, m)) != hipSuccess) return x;
+  // this job's error word: cleared before its first sweep, read back after its last
+  if ((x = solve_sync_begin(st.sync, st.s)) != hipSuccess) return x;
   std::memcpy(hx, j.req.b, nm * 8);
   if ((x = hipMemcpyAsync(db, hx, nm * 8, hipMemcpyHostToDevice, st.s)) != hipSuccess) return x;
   if (e->prec == LLH_PREC_F64) {
@@ -270,12 +297,12 @@ hipError_t enqueue_solve(llh_server* s, Job& j) {
     if ((x = launch_fill_i32(reinterpret_cast<int32_t*>(dx), 0, 2 * (int64_t)nm, st.s)) != hipSuccess) return x;
     if ((x = launch_axpy_s2d(n, m, dd, m, dx, m, st.s)) != hipSuccess) return x;
     for (int it = 0; it < s->cfg.ir_iterations; ++it) {
-      if ((x = launch_symm_residual(n, m, e->a, n, dx, m, db, m, dr, m, st.s)) != hipSuccess) return x;
+      if ((x = launch_symm_residual(n, m, e->a_own, n, dx, m, db, m, dr, m, st.s)) != hipSuccess) return x;
       if ((x = launch_convert_d2s(n, m, dr, m, dd, m, false, st.s)) != hipSuccess) return x;
       if ((x = solve_with<float>(st, e, m, dd)) != hipSuccess) return x;
       if ((x = launch_axpy_s2d(n, m, dd, m, dx, m, st.s)) != hipSuccess) return x;
     }
-    if ((x = launch_symm_residual(n, m, e->a, n, dx, m, db, m, dr, m, st.s)) != hipSuccess) return x;
+    if ((x = launch_symm_residual(n, m, e->a_own, n, dx, m, db, m, dr, m, st.s)) != hipSuccess) return x;
     if ((x = launch_colmax(n, m, dr, m, dn, st.s)) != hipSuccess) return x;
     if ((x = launch_colmax(n, m, dx, m, dn + m, st.s)) != hipSuccess) return x;
     if ((x = hipMemcpyAsync(hx + nm, dn, 2 * m * 8, hipMemcpyDeviceToHost, st.s)) != hipSuccess) return x;
@@ -284,16 +311,16 @@ hipError_t enqueue_solve(llh_server* s, Job& j) {
       return x;
   }
   if ((x = hipMemcpyAsync(hx, dx, nm * 8, hipMemcpyDeviceToHost, st.s)) != hipSuccess) return x;
-  if ((x = hipMemcpyAsync(reinterpret_cast<int32_t*>(hx + nm + 2 * m), e->d_info, 4, hipMemcpyDeviceToHost,
-                          st.s)) != hipSuccess)
-    return x;
+  int32_t* info = stage_info(hx, nm, m);
+  if ((x = hipMemcpyAsync(info, e->d_info, 4, hipMemcpyDeviceToHost, st.s)) != hipSuccess) return x;
+  if ((x = hipMemcpyAsync(info + 1, st.sync.words + 1, 4, hipMemcpyDeviceToHost, st.s)) != hipSuccess) return x;
   return hipEventRecord(j.done, st.s);
 }
 
-llh_status fail_job(llh_server* s, Job& j, llh_status code, const char* msg) {
+llh_status fail_job(llh_server* s, Job& j, llh_status code, const char* msg, hipError_t hip = hipSuccess) {
   llh_result r{};
   r.status = code;
-  r.failed_row = -1;
+  r.failed_row = code == LLH_ERR_HIP ? (int64_t)hip : -1;
   r.submit_s = j.t_submit;
   r.admit_s = j.t_admit;
   r.done_s = now_s();
@@ -313,7 +340,8 @@ void finish(llh_server* s, Job& j) {
   const size_t nm = (size_t)e->n * j.req.m;
   double* hx = static_cast<double*>(j.host);
   // a job whose enqueue failed never wrote its staging block
-  const int32_t info = j.error == hipSuccess ? *reinterpret_cast<int32_t*>(hx + nm + 2 * j.req.m) : kNoFail;
+  const int32_t info = j.error == hipSuccess ? stage_info(hx, nm, j.req.m)[0] : kNoFail;
+  const int32_t solve_err = j.error == hipSuccess ? stage_info(hx, nm, j.req.m)[1] : 0;
   llh_result r{};
   r.cache_hit = j.hit;
   r.batched = j.batched;
@@ -331,6 +359,11 @@ void finish(llh_server* s, Job& j) {
     r.failed_row = info;
     e->failed = true;
     e->failed_row = info;
+  } else if (solve_err != 0) {
+    // a sweep's bounded wait gave up: x is unfinished, the factor is sound
+    r.status = LLH_ERR_HIP;
+    r.failed_row = 0;
+    set_error(&s->last_error, LLH_ERR_HIP, 0, 0, -1, "triangular solve: a block wait exceeded its bound", 0);
   } else {
     r.status = LLH_OK;
     std::memcpy(j.req.x, hx, nm * 8);
@@ -413,17 +446,21 @@ bool admit(llh_server* s, Job& j) {
     bool okm = hipMallocAsync(&e->l, n2 * es_of(prec), st) == hipSuccess &&
                hipMallocAsync(&e->dinv, (size_t)dinv_elems(r.n) * es_of(prec), st) == hipSuccess &&
                hipMallocAsync(reinterpret_cast<void**>(&e->d_info), 256, st) == hipSuccess;
-    if (okm && !r.a_on_device) okm = hipMallocAsync(reinterpret_cast<void**>(&e->a_own), n2 * 8, st) == hipSuccess;
-    e->a = r.a_on_device ? r.a : e->a_own;
-    e->ready = get_event(s);
+    if (okm && prec == LLH_PREC_F32)
+      okm = hipMallocAsync(reinterpret_cast<void**>(&e->a_own), n2 * 8, st) == hipSuccess;
+    const hipError_t xev = okm ? get_event(s, &e->ready) : hipSuccess;
     e->bytes = eb;
     s->used += eb;
-    if (!okm) {
+    if (!okm || xev != hipSuccess) {
+      (void)hipGetLastError();
       free_entry(s, e, st);
       dev_free(st, j.dev);
       s->used -= jb;
       s->pinned.put(j.host, j.host_cls);
-      fail_job(s, j, LLH_ERR_OUT_OF_MEMORY, "factor buffers");
+      if (!okm)
+        fail_job(s, j, LLH_ERR_OUT_OF_MEMORY, "factor buffers");
+      else
+        fail_job(s, j, LLH_ERR_HIP, "factor event", xev);
       return true;
     }
     auto old = s->cache.find(r.matrix_id);
@@ -449,11 +486,14 @@ bool admit(llh_server* s, Job& j) {
 
 void launch_small_step(llh_server* s, std::vector<Job*>& batch) {
   // One launch of the exact-order kernel over every new small system, on the
-  // stream their buffers were allocated on.
+  // stream their buffers were allocated on.  Any failed enqueue fails every job
+  // of the step (and marks its entries failed): none of them reads a status
+  // word or a solution that nothing wrote.
   const int si = 0;
   StreamState& st = s->streams[si];
   std::vector<SmallDesc<double>> descs;
   int nmax = 0;
+  hipError_t x = hipSuccess;
   for (Job* j : batch) {
     j->stream = si;
     Entry* e = j->entry;
@@ -461,48 +501,75 @@ void launch_small_step(llh_server* s, std::vector<Job*>& batch) {
     const size_t nm = (size_t)n * m;
     double* hx = static_cast<double*>(j->host);
     double* dx = reinterpret_cast<double*>(static_cast<char*>(j->dev) + round_up(nm * 8, 256));
-    if (j->req.a_on_device)
-      (void)hipMemcpyAsync(e->l, j->req.a, (size_t)n * n * 8, hipMemcpyDeviceToDevice, st.s);
-    else
-      (void)hipMemcpyAsync(e->l, j->req.a, (size_t)n * n * 8, hipMemcpyHostToDevice, st.s);
-    if (!j->req.a_on_device)
-      (void)hipMemcpyAsync(e->a_own, j->req.a, (size_t)n * n * 8, hipMemcpyHostToDevice, st.s);
+    const hipMemcpyKind kind = j->req.a_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (x == hipSuccess) x = hipMemcpyAsync(e->l, j->req.a, (size_t)n * n * 8, kind, st.s);
     std::memcpy(hx, j->req.b, nm * 8);
-    (void)hipMemcpyAsync(dx, hx, nm * 8, hipMemcpyHostToDevice, st.s);
+    stage_info(hx, nm, m)[1] = 0;  // the batched kernel has no flag waits
+    if (x == hipSuccess) x = hipMemcpyAsync(dx, hx, nm * 8, hipMemcpyHostToDevice, st.s);
     descs.push_back({static_cast<double*>(e->l), dx, n, m, n, m, e->d_info});
     nmax = std::max(nmax, n);
   }
   // descriptors through a pinned staging block
+  const size_t dbytes = descs.size() * sizeof(SmallDesc<double>);
   size_t cls = 0;
-  void* hd = s->pinned.get(descs.size() * sizeof(SmallDesc<double>), &cls);
+  void* hd = nullptr;
   void* dd = nullptr;
-  const bool ok = hd != nullptr && hipMallocAsync(&dd, descs.size() * sizeof(SmallDesc<double>), st.s) == hipSuccess;
-  if (ok) {
-    std::memcpy(hd, descs.data(), descs.size() * sizeof(SmallDesc<double>));
-    (void)hipMemcpyAsync(dd, hd, descs.size() * sizeof(SmallDesc<double>), hipMemcpyHostToDevice, st.s);
-    (void)launch_potrf_potrs_exact_batched<double>(static_cast<SmallDesc<double>*>(dd), (int)descs.size(), nmax, 1,
-                                                   st.s);
-    dev_free(st.s, dd);
+  if (x == hipSuccess) {
+    if (s->inject_staging.load() > 0) {
+      s->inject_staging.fetch_sub(1);
+      x = hipErrorOutOfMemory;
+    } else if ((hd = s->pinned.get(dbytes, &cls)) == nullptr) {
+      x = hipErrorOutOfMemory;
+    } else {
+      x = hipMallocAsync(&dd, dbytes, st.s);
+      if (x != hipSuccess) dd = nullptr;
+    }
   }
-  hipEvent_t ev = get_event(s);
-  auto* stage = new StepStage{ev, hd, cls, (int)batch.size()};
+  if (x == hipSuccess) {
+    std::memcpy(hd, descs.data(), dbytes);
+    x = hipMemcpyAsync(dd, hd, dbytes, hipMemcpyHostToDevice, st.s);
+  }
+  if (x == hipSuccess)
+    x = launch_potrf_potrs_exact_batched<double>(static_cast<SmallDesc<double>*>(dd), (int)descs.size(), nmax, 1,
+                                                 st.s);
+  if (dd) dev_free(st.s, dd);
+  if (x != hipSuccess) (void)hipGetLastError();
   for (Job* j : batch) {
     Entry* e = j->entry;
     const size_t nm = (size_t)e->n * j->req.m;
     double* hx = static_cast<double*>(j->host);
     double* dx = reinterpret_cast<double*>(static_cast<char*>(j->dev) + round_up(nm * 8, 256));
-    (void)hipMemcpyAsync(hx, dx, nm * 8, hipMemcpyDeviceToHost, st.s);
-    (void)hipMemcpyAsync(reinterpret_cast<int32_t*>(hx + nm + 2 * j->req.m), e->d_info, 4, hipMemcpyDeviceToHost,
-                         st.s);
+    if (x == hipSuccess) x = hipMemcpyAsync(hx, dx, nm * 8, hipMemcpyDeviceToHost, st.s);
+    if (x == hipSuccess) x = hipMemcpyAsync(stage_info(hx, nm, j->req.m), e->d_info, 4, hipMemcpyDeviceToHost, st.s);
+  }
+  if (x != hipSuccess) (void)hipGetLastError();
+  hipEvent_t ev = nullptr;
+  const hipError_t xev = get_event(s, &ev);
+  auto* stage = new StepStage{ev, hd, cls, (int)batch.size()};
+  for (Job* j : batch) {
+    Entry* e = j->entry;
+    if (x != hipSuccess) {
+      j->error = x;
+      e->failed = true;  // no later request may use its factor
+      e->error = x;
+    }
     (void)hipEventRecord(e->ready, st.s);
     e->in_step = false;
     j->batched = true;
     j->done = ev;
     j->step = stage;
   }
-  (void)hipEventRecord(ev, st.s);
+  if (xev != hipSuccess || hipEventRecord(ev, st.s) != hipSuccess) {
+    // no event to poll: wait for the step here; the jobs are then complete
+    (void)hipGetLastError();
+    const hipError_t xs = hipStreamSynchronize(st.s);
+    for (Job* j : batch) {
+      j->settled = true;
+      if (j->error == hipSuccess && xs != hipSuccess) j->error = xs;
+    }
+  }
   // the staging block and the shared event are recycled after the step's last
-  // job finishes (finish_step)
+  // job finishes
   std::lock_guard<std::mutex> g(s->mu);
   s->stats.batched += batch.size();
   s->stats.steps++;
@@ -534,13 +601,13 @@ void worker_loop(llh_server* s) {
     bool progressed = false;
     // completions
     for (auto it = s->jobs.begin(); it != s->jobs.end();) {
-      if (hipEventQuery(it->done) == hipSuccess) {
+      if (it->settled || (it->done != nullptr && hipEventQuery(it->done) == hipSuccess)) {
         StepStage* stage = it->step;
         finish(s, *it);
         it = s->jobs.erase(it);
         if (stage && --stage->remaining == 0) {
           if (stage->host) s->pinned.put(stage->host, stage->cls);
-          s->event_pool.push_back(stage->done);
+          if (stage->done) s->event_pool.push_back(stage->done);
           delete stage;
         }
         progressed = true;
@@ -590,7 +657,12 @@ void worker_loop(llh_server* s) {
           if (x == hipSuccess) x = hipEventRecord(q.entry->ready, st.s);  // later users wait for the inverses too
         }
       }
-      q.done = get_event(s);
+      hipEvent_t done = nullptr;
+      const hipError_t xev = get_event(s, &done);
+      q.done = done;
+      // a hit on an entry whose batched step failed to enqueue
+      if (x == hipSuccess && q.entry->failed && q.entry->error != hipSuccess) x = q.entry->error;
+      if (x == hipSuccess) x = xev;
       if (x == hipSuccess) x = enqueue_solve(s, q);
       if (x != hipSuccess) {
         q.error = x;
@@ -598,9 +670,15 @@ void worker_loop(llh_server* s) {
         if (!q.hit) {
           // the factor may be incomplete: no later request may use the entry
           q.entry->failed = true;
+          q.entry->error = x;
           (void)hipEventRecord(q.entry->ready, st.s);
         }
-        (void)hipEventRecord(q.done, st.s);
+        if (q.done == nullptr || hipEventRecord(q.done, st.s) != hipSuccess) {
+          // no event to poll: wait for what was enqueued; the job is then complete
+          (void)hipGetLastError();
+          (void)hipStreamSynchronize(st.s);
+          q.settled = true;
+        }
       }
     }
     if (!small.empty()) launch_small_step(s, small);
@@ -706,6 +784,17 @@ void llh_server_get_stats(llh_server* s, llh_server_stats* out) {
   if (s == nullptr || s->magic != llh_server::kMagic || out == nullptr) return;
   std::lock_guard<std::mutex> g(s->mu);
   *out = s->stats;  // the worker publishes the memory figures into stats under mu
+}
+
+llh_status llh_server_inject(llh_server* s, int kind, int64_t value, llh_error* err) {
+  if (s == nullptr || s->magic != llh_server::kMagic) return invalid(err, "null or destroyed server");
+  switch (kind) {
+    case LLH_SERVER_INJECT_STAGING: s->inject_staging.store((int)value); break;
+    case LLH_SERVER_INJECT_EVENT: s->inject_event.store((int)value); break;
+    case LLH_SERVER_INJECT_SOLVE_SPIN: s->spin_limit.store(value == 0 ? kSpinLimit : value); break;
+    default: return invalid(err, "unknown injection");
+  }
+  return ok(err);
 }
 
 void llh_server_evict_all(llh_server* s) {

@@ -72,6 +72,7 @@ class ServerStats(ctypes.Structure):
 
 
 PREC = {"f64": 0, "f32": 1, "f32ir": 2, "f16": 3}
+SERVER_INJECT = {"staging": 1, "event": 2, "solve_spin": 3}
 FALLBACK_F64 = 1
 
 KERNEL_CLASSES = {"trailing_syrk": 0, "panel_update": 1, "diag_factor": 2, "panel_trsm": 3,
@@ -90,6 +91,7 @@ def _declare(lib):
     lib.llh_context_set_profiling.argtypes = [_vp, _int, _err]
     lib.llh_context_set_profiling_classes.argtypes = [_vp, ctypes.c_uint32, _err]
     lib.llh_context_kernel_stats.argtypes = [_vp, _int, ctypes.POINTER(KernelStats), _err]
+    lib.llh_context_set_solve_spin_limit.argtypes = [_vp, ctypes.c_int64, _err]
     lib.llh_factor_create.argtypes = [_vp, _int, _sz, _vp, _sz, ctypes.c_int32, ctypes.POINTER(_vp), _err]
     lib.llh_factor_solve.argtypes = [_vp, _vp, _sz, _vp, _sz, _vp, _sz, ctypes.POINTER(RefineOpts),
                                      ctypes.POINTER(RefineInfo), _err]
@@ -120,6 +122,7 @@ def _declare(lib):
     lib.llh_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(Result), _err]
     lib.llh_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
     lib.llh_server_get_stats.restype = None
+    lib.llh_server_inject.argtypes = [_vp, _int, ctypes.c_int64, _err]
     lib.llh_server_evict_all.argtypes = [_vp]
     lib.llh_server_evict_all.restype = None
     lib.llh_server_destroy.argtypes = [_vp]

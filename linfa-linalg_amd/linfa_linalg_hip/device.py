@@ -45,6 +45,15 @@ def _call(name, t, *args):
     _raise(getattr(c._lib, f"llh_dev_{name}_{_sfx(t)}")(c.handle, *args, ctypes.byref(err)), err)
 
 
+def set_solve_spin_limit(limit):
+    """Bound on the one-launch solve's flag waits for this stream's context
+    (negative: every wait gives up at once, to test the error path; 0: the
+    default)."""
+    c = context()
+    err = _abi.LlhError()
+    _raise(c._lib.llh_context_set_solve_spin_limit(c.handle, int(limit), ctypes.byref(err)), err)
+
+
 def potrf(a, zero_upper=False):
     """In-place lower Cholesky of a square tensor (dirty unless zero_upper)."""
     _check(a)
@@ -361,6 +370,13 @@ class Server:
 
     def evict_all(self):
         self._c._lib.llh_server_evict_all(self._ptr)
+
+    def inject(self, kind, value):
+        """Host-forced failures for tests (llh_server_inject): kind is
+        "staging", "event" or "solve_spin"."""
+        err = _abi.LlhError()
+        _raise(self._c._lib.llh_server_inject(self._ptr, _abi.SERVER_INJECT[kind], int(value),
+                                              ctypes.byref(err)), err)
 
     def close(self):
         if self._ptr:
