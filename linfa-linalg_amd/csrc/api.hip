@@ -649,6 +649,12 @@ llh_status llh_context_create(int device, llh_context** out, llh_error* err) {
     return hip_fail(err, e, "context create");
   }
   ctx->own_stream = true;
+  // Set-up that later calls must not repeat: the factor's lookahead streams
+  // (without them the factor runs on one stream), and this device's copy of the
+  // allocation-free entry point's static scratch (that entry point reports a
+  // failure here when it is called).
+  (void)ctx_lookahead(ctx);
+  (void)noalloc_init(dev, ctx->stream);
   *out = ctx;
   return ok(err);
 }

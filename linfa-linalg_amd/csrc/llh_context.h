@@ -54,4 +54,7 @@ hipError_t ctx_sync(llh_context* ctx, int n, int m, SolveSync* sync);
 const Lookahead* ctx_lookahead(llh_context* ctx);
 // Enqueues the read-back of the solve error word into h_info[2].
 hipError_t ctx_read_solve_error(llh_context* ctx);
+// Locates `device`'s copy of the allocation-free entry point's static scratch
+// (noalloc.hip); runs once per device, with that device current.
+hipError_t noalloc_init(int device, hipStream_t s);
 }  // namespace llh

@@ -144,6 +144,7 @@ def _declare(lib):
         f("dev_solvec").argtypes = [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _err]
         f("dev_orthonormalize").argtypes = [_vp, _sz, _sz, _vp, _sz, _vp, _sz, _err]
         f("dev_apply_constraints").argtypes = [_vp, _sz, _sz, _vp, _sz, _sz, _vp, _sz, _vp, _sz, _err]
+    lib.llh_dev_solvec_noalloc_d.argtypes = [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _err]
 
 
 def _share_torch_runtime():
