@@ -222,6 +222,27 @@ llh_status llh_dev_solvec_noalloc_d(llh_context* ctx, size_t n, double* a, size_
                                     size_t ldb, llh_error* err);
 
 /* ---------------------------------------------------------------------------
+ * Seeded synthetic workloads (no counterpart in the reference; for benchmarks
+ * and full-size parity tests).  Values come from counters, so a CPU restatement
+ * reproduces every one bit for bit: element (i, j) of a rows x cols matrix drawn
+ * with (seed, stream) is value(key, i * cols + j), key = SplitMix64 output 0 from
+ * the state seed + stream * 0x9E3779B97F4A7C15, draw c = SplitMix64 output c
+ * from the state key.  LLH_GEN_UNIFORM: 2 u - 1 in [-1, 1), u = top 53 bits /
+ * 2^53 of draw idx.  LLH_GEN_NORMAL: Marsaglia's polar method on draws
+ * 2c, 2c + 1 (c = 64 idx + attempt), v1 sqrt(-2 ln(s) / s), ln(s) from a fixed
+ * atanh series of correctly rounded operations.
+ * ------------------------------------------------------------------------- */
+enum { LLH_GEN_NORMAL = 0, LLH_GEN_UNIFORM = 1 };
+/* x (rows x cols, row-major ldx, device f64) := the (seed, stream) draw. */
+llh_status llh_dev_gen_matrix_d(llh_context* ctx, int dist, uint64_t seed, uint64_t stream, size_t rows,
+                                size_t cols, double* x, size_t ldx, llh_error* err);
+/* a (n x n, lda) := B B^T / n + lam I with B = the (seed, stream 0) draw, written
+ * to b_work (n x n, contiguous): a_ij = (sum_k b_ik b_jk in k order, each
+ * product and sum rounded) / n, plus lam on the diagonal; both triangles. */
+llh_status llh_dev_gen_spd_d(llh_context* ctx, int dist, uint64_t seed, size_t n, double lam, double* a,
+                             size_t lda, double* b_work, llh_error* err);
+
+/* ---------------------------------------------------------------------------
  * Mixed precision with iterative refinement (no counterpart in the reference;
  * for callers that want an f64-accurate SPD solve at FP32-MFMA speed).
  *

@@ -144,6 +144,8 @@ def _declare(lib):
         f("dev_solvec").argtypes = [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _err]
         f("dev_orthonormalize").argtypes = [_vp, _sz, _sz, _vp, _sz, _vp, _sz, _err]
         f("dev_apply_constraints").argtypes = [_vp, _sz, _sz, _vp, _sz, _sz, _vp, _sz, _vp, _sz, _err]
+    lib.llh_dev_gen_matrix_d.argtypes = [_vp, _int, ctypes.c_uint64, ctypes.c_uint64, _sz, _sz, _vp, _sz, _err]
+    lib.llh_dev_gen_spd_d.argtypes = [_vp, _int, ctypes.c_uint64, _sz, ctypes.c_double, _vp, _sz, _vp, _err]
     lib.llh_dev_solvec_noalloc_d.argtypes = [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _err]
 
 

@@ -54,6 +54,36 @@ def set_solve_spin_limit(limit):
     _raise(c._lib.llh_context_set_solve_spin_limit(c.handle, int(limit), ctypes.byref(err)), err)
 
 
+GEN_NORMAL, GEN_UNIFORM = 0, 1
+
+
+def gen_matrix(dist, seed, stream, rows, cols):
+    """A new rows x cols float64 CUDA tensor holding the seeded draw
+    (llh_dev_gen_matrix_d; dist GEN_NORMAL or GEN_UNIFORM)."""
+    import torch
+    x = torch.empty((rows, cols), dtype=torch.float64, device="cuda")
+    c = context()
+    err = _abi.LlhError()
+    _raise(c._lib.llh_dev_gen_matrix_d(c.handle, dist, seed, stream, rows, cols, ctypes.c_void_p(x.data_ptr()),
+                                       cols, ctypes.byref(err)), err)
+    return x
+
+
+def gen_spd(dist, seed, n, lam, out=None):
+    """A = B B^T / n + lam I with B the (seed, stream 0) draw (llh_dev_gen_spd_d),
+    into `out` (n x n float64 CUDA tensor) or a new tensor."""
+    import torch
+    a = out if out is not None else torch.empty((n, n), dtype=torch.float64, device="cuda")
+    _check(a)
+    w = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    c = context()
+    err = _abi.LlhError()
+    _raise(c._lib.llh_dev_gen_spd_d(c.handle, dist, seed, n, float(lam), ctypes.c_void_p(a.data_ptr()), a.stride(0),
+                                    ctypes.c_void_p(w.data_ptr()), ctypes.byref(err)), err)
+    del w
+    return a
+
+
 def potrf(a, zero_upper=False):
     """In-place lower Cholesky of a square tensor (dirty unless zero_upper)."""
     _check(a)

@@ -66,6 +66,22 @@ ORC_DECL(double, d)
 ORC_DECL(float, s)
 #undef ORC_DECL
 
+/* llh_oracle_par.c: the seeded workload generator and the multi-threaded,
+ * bit-identical restatement of solvec for contiguous row-major f64 matrices. */
+enum { ORC_GEN_NORMAL = 0, ORC_GEN_UNIFORM = 1 };
+uint64_t orc_splitmix64(uint64_t state, uint64_t c); /* output c (0-based) of SplitMix64 from state */
+uint64_t orc_gen_key(uint64_t seed, uint64_t stream);
+double orc_log(double s);
+double orc_gen_value(int dist, uint64_t key, uint64_t idx);
+void orc_gen_matrix_d(int dist, uint64_t seed, uint64_t stream, int64_t rows, int64_t cols, double* x,
+                      int64_t ldx);
+void orc_gen_rows_d(int dist, uint64_t seed, uint64_t stream, int64_t cols, int64_t row0, int64_t nrows,
+                    double* x);
+void orc_gen_spd_d(int dist, uint64_t seed, int64_t n, double lam, double* a, int64_t lda, double* b_work);
+int orc_par_cholesky_dirty_d(int64_t n, double* a, int64_t lda, int64_t* fail);
+void orc_par_solve_lower_upper_d(int64_t n, const double* a, int64_t lda, int64_t m, double* b, int64_t ldb);
+int orc_par_solvec_d(int64_t n, double* a, int64_t lda, int64_t m, double* b, int64_t ldb, int64_t* fail);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,46 @@ def test_f32_ir_matches_f64_oracle(n, lam):
     assert abs(info["anorm"] - anorm) <= 1e-13 * anorm, (info["anorm"], anorm)
 
 
+@pytest.mark.parametrize("n", [1500, 3000])
+def test_f32_ir_hardest_lambda(n):
+    """The hardest refinement case of the mixed-precision workload, lambda =
+    1e-6 (cond ~4e6): the f32-updated factor still refines to f64 backward
+    accuracy; the solution agrees with the oracle's to the measured
+    cond * eps64 (both are backward-stable f64 solutions)."""
+    from linfa_linalg_hip import device as dev
+    import torch
+    lam = 1e-6
+    a = spd_normal(n, lam, n + 11)
+    b = np.random.default_rng(n + 12).standard_normal((n, 16))
+    x = torch.zeros((n, 16), dtype=torch.float64, device="cuda")
+    info = dev.solve_mixed(tt(a), tt(b), x, prec="f32", fallback=False)
+    assert info["converged"] and not info["fell_back"] and info["precision_used"] == "f32", info
+    cond = np.linalg.cond(a)
+    assert cond > 1e5, cond  # the workload is as ill-conditioned as intended
+    x_ref = orc.solvec(np.array(a), b)
+    err = np.linalg.norm(x.cpu().numpy() - x_ref) / np.linalg.norm(x_ref)
+    assert err <= max(1e-12, 50 * cond * np.finfo(np.float64).eps), (err, cond, info)
+    assert info["residual"] <= 10 * np.sqrt(n) * np.finfo(np.float64).eps
+
+
+@pytest.mark.parametrize("n", [1600, 3000])
+def test_f16_ir_lambda_1e2(n):
+    """FP16+IR at the mixed-precision workload's lambda = 1e-2 (cond ~4e2):
+    more refinement steps than at lambda = 1, same f64 accuracy."""
+    from linfa_linalg_hip import device as dev
+    import torch
+    a = spd_normal(n, 1e-2, n + 13)
+    b = np.random.default_rng(n + 14).standard_normal((n, 16))
+    x = torch.zeros((n, 16), dtype=torch.float64, device="cuda")
+    info = dev.solve_mixed(tt(a), tt(b), x, prec="f16", fallback=False)
+    assert info["converged"] and not info["fell_back"] and info["precision_used"] == "f16", info
+    cond = np.linalg.cond(a)
+    x_ref = orc.solvec(np.array(a), b)
+    err = np.linalg.norm(x.cpu().numpy() - x_ref) / np.linalg.norm(x_ref)
+    assert err <= max(1e-12, 50 * cond * np.finfo(np.float64).eps), (err, cond, info)
+    assert info["residual"] <= 10 * np.sqrt(n) * np.finfo(np.float64).eps
+
+
 def test_f64_factor_and_reuse():
     from linfa_linalg_hip import device as dev
     import torch
