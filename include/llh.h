@@ -185,6 +185,12 @@ LLH_PLAIN_DECL(float, s)
   /* solvec_inplace on device memory (cholesky.rs:136-144): A := dirty factor, B := X. */ \
   llh_status llh_dev_solvec_##sfx(llh_context* ctx, size_t n, T* a, size_t lda, size_t m,  \
                                   T* b, size_t ldb, llh_error* err);                      \
+  /* invc_inplace on device memory (cholesky.rs:166-183): A := dirty factor, out (n x n,  \
+     ldo >= n) := A^-1, both triangles.  Above the exact-order kernel's bound the inverse   \
+     is L^-T L^-1 formed on MFMA (about 1.5 n^3 flops, against the 2 n^3 + n^3/3 of the     \
+     reference's solve with the identity); out is untouched after NotPositiveDefinite. */  \
+  llh_status llh_dev_invc_##sfx(llh_context* ctx, size_t n, T* a, size_t lda, T* out,      \
+                                size_t ldo, llh_error* err);                               \
   /* LOBPCG's orthonormalize(v), lobpcg/algorithm.rs:75-97 (private to the LOBPCG        \
      module, the Cholesky path's own consumer): G = V^T V, L = cholesky_into(G), then     \
      V := (L^-1 V^T)^T.  v: n x k row-major on the device (ldv >= k), overwritten with    \

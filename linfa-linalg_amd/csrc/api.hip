@@ -374,6 +374,34 @@ llh_status solve_on_device(llh_context* ctx, size_t n, T* da, size_t m, T* db, l
   return LLH_OK;
 }
 
+// A^-1 of the device matrix da (n x n, lda) into out (ldo): the factor in place
+// (da := dirty factor), then, when it succeeded, W = L^-1 and W^T W on MFMA
+// (potri_enqueue) -- about 1.5 n^3 flops where the reference's solvec on the
+// identity (cholesky.rs:178-182) takes 2 n^3 + n^3 / 3.  The status is left in
+// ctx->h_info[0]; out is untouched after NotPositiveDefinite.  n must exceed
+// the exact-order kernel's bound (below it the caller takes that kernel).
+template <typename T>
+llh_status invert_on_device(llh_context* ctx, size_t n, T* da, size_t lda, T* out, size_t ldo, llh_error* err) {
+  void* dv = nullptr;
+  CHECK_HIP(ctx_dinv(ctx, (int)n, sizeof(T), &dv), "inverse buffer");
+  T* dinv = static_cast<T*>(dv);
+  void* wv = nullptr;
+  const size_t wel = n * n, tel = (size_t)potri_tmp_elems((int)n);
+  CHECK_HIP(ctx_buffer(ctx, 3, (wel + tel) * sizeof(T), &wv), "inverse work buffer");
+  T* w = static_cast<T*>(wv);
+  CHECK_HIP(potrf_enqueue<T>(da, (int64_t)lda, (int)n, ctx->d_info, false, dinv, ctx->stream, ctx->profiler(), false,
+                             ctx_lookahead(ctx)),
+            "potrf");
+  CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
+            "status");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  ctx->h_info[2] = 0;  // no one-launch solve runs here: no wait to report
+  if (ctx->h_info[0] != kNoFail) return LLH_OK;
+  CHECK_HIP(potri_enqueue<T>((int)n, da, (int64_t)lda, dinv, w, (int64_t)n, out, (int64_t)ldo, w + wel, ctx->stream),
+            "inverse");
+  return LLH_OK;
+}
+
 template <typename T>
 llh_status solvec_plain(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdiff_t rs, ptrdiff_t cs,
                         size_t b_rows, size_t b_cols, T* b, ptrdiff_t brs, ptrdiff_t bcs, T* inv_out,
@@ -409,13 +437,18 @@ llh_status solvec_plain(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdif
   T* da = static_cast<T*>(dav);
   T* db = static_cast<T*>(dbv);
   CHECK_HIP(hipMemcpyAsync(da, ha.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
-  if (invert) {
-    CHECK_HIP(launch_set_identity<T>(db, (int64_t)n, (int)n, ctx->stream), "identity");  // cholesky.rs:179
+  llh_status st = LLH_OK;
+  if (invert && (int)n > small_max<T>()) {
+    st = invert_on_device<T>(ctx, n, da, n, db, n, err);
   } else {
-    gather(b, brs, bcs, n, m, hb.data());
-    CHECK_HIP(hipMemcpyAsync(db, hb.data(), n * m * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+    if (invert) {
+      CHECK_HIP(launch_set_identity<T>(db, (int64_t)n, (int)n, ctx->stream), "identity");  // cholesky.rs:179
+    } else {
+      gather(b, brs, bcs, n, m, hb.data());
+      CHECK_HIP(hipMemcpyAsync(db, hb.data(), n * m * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+    }
+    st = solve_on_device<T>(ctx, n, da, m, db, err);
   }
-  llh_status st = solve_on_device<T>(ctx, n, da, m, db, err);
   if (st != LLH_OK) return st;
   CHECK_HIP(hipMemcpyAsync(ha.data(), da, n * n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
   CHECK_HIP(hipMemcpyAsync(hb.data(), db, n * m * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
@@ -508,6 +541,35 @@ llh_status dev_solvec(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T*
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
   if (ctx->h_info[0] != kNoFail) return not_pd(err, ctx->h_info[0]);
   if (ctx->h_info[2] != 0) return solve_stalled(err);
+  return ok(err);
+}
+
+// invc_inplace on device memory (cholesky.rs:166-183): a := dirty factor,
+// out := A^-1 (both triangles).  Systems within the exact-order kernel's bound
+// solve the identity in the reference's operation order; larger ones take the
+// MFMA inverse (invert_on_device).
+template <typename T>
+llh_status dev_invc(llh_context* ctx, size_t n, T* a, size_t lda, T* out, size_t ldo, llh_error* err) {
+  if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
+  if (n == 0) return ok(err);
+  if (a == nullptr || out == nullptr || lda < n || ldo < n) return invalid(err, "bad matrix arguments");
+  if (n > 0x7fffffff) return invalid(err, "matrix too large");
+  if ((int)n <= small_max<T>()) {
+    CHECK_HIP(launch_set_identity<T>(out, (int64_t)ldo, (int)n, ctx->stream), "identity");
+    SmallDesc<T> desc{a, out, (int64_t)lda, (int64_t)ldo, (int32_t)n, (int32_t)n, ctx->d_info};
+    void* ddv = nullptr;
+    CHECK_HIP(ctx_buffer(ctx, 2, sizeof(desc), &ddv), "descriptor buffer");
+    CHECK_HIP(hipMemcpyAsync(ddv, &desc, sizeof(desc), hipMemcpyHostToDevice, ctx->stream), "descriptor");
+    CHECK_HIP(launch_potrf_potrs_exact_batched<T>(static_cast<SmallDesc<T>*>(ddv), 1, (int)n, 1, ctx->stream),
+              "small factor+solve");
+    CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
+              "status");
+  } else {
+    llh_status st = invert_on_device<T>(ctx, n, a, lda, out, ldo, err);
+    if (st != LLH_OK) return st;
+  }
+  CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  if (ctx->h_info[0] != kNoFail) return not_pd(err, ctx->h_info[0]);
   return ok(err);
 }
 
@@ -799,6 +861,10 @@ llh_status llh_context_kernel_stats(llh_context* ctx, int kernel_class, llh_kern
   llh_status llh_dev_solvec_##sfx(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T* b,         \
                                   size_t ldb, llh_error* err) {                                         \
     return dev_solvec<T>(ctx, n, a, lda, m, b, ldb, err);                                               \
+  }                                                                                                     \
+  llh_status llh_dev_invc_##sfx(llh_context* ctx, size_t n, T* a, size_t lda, T* out, size_t ldo,        \
+                                llh_error* err) {                                                       \
+    return dev_invc<T>(ctx, n, a, lda, out, ldo, err);                                                  \
   }                                                                                                     \
   llh_status llh_dev_orthonormalize_##sfx(llh_context* ctx, size_t n, size_t k, T* v, size_t ldv, T* r,  \
                                           size_t ldr, llh_error* err) {                                 \

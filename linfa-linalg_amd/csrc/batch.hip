@@ -61,6 +61,8 @@ struct llh_batch_plan {
   std::vector<std::pair<int, int>> bwd;  // StepWork ranges of the back-solve steps
   int res_off = 0, nres = 0;             // StepWork range of the residual blocks (refined mode)
   int nclasses = 0;                      // small-system launches per pass
+  int med_first = 0, nmed = 0;           // items of the blocked (medium) bucket
+  bool lds_solve = false;                // every medium n <= kSolveLdsMaxN: one substitution launch
   int ncounters = 0;
   size_t linv_elems = 0;
 
@@ -141,12 +143,19 @@ hipError_t record_pass(llh_batch_plan* p, hipStream_t s, int pass) {
       if ((e = launch_grp_diag_fast<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
       if ((e = launch_grp_panel<T>(base, items, tw + st.panel_off, st.npanel, s)) != hipSuccess) return e;
     }
-    if ((e = launch_grp_fwd<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
+    if (!p->lds_solve)
+      if ((e = launch_grp_fwd<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
     if (pass == 0)
       if ((e = launch_grp_syrk<T>(base, items, tw + st.syrk_off, st.nsyrk, s)) != hipSuccess) return e;
   }
-  for (const auto& bw : p->bwd)
-    if ((e = launch_grp_bwd<T>(base, items, sw + bw.first, bw.second, s)) != hipSuccess) return e;
+  if (p->lds_solve) {
+    // the whole forward and back substitution of every medium system, one
+    // workgroup each, after its factor
+    if ((e = launch_grp_solve<T>(base, items + p->med_first, p->nmed, s)) != hipSuccess) return e;
+  } else {
+    for (const auto& bw : p->bwd)
+      if ((e = launch_grp_bwd<T>(base, items, sw + bw.first, bw.second, s)) != hipSuccess) return e;
+  }
   if ((e = hipEventRecord(p->join, p->side)) != hipSuccess) return e;
   return hipStreamWaitEvent(s, p->join, 0);
 }
@@ -234,6 +243,9 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
     maxn = std::max(maxn, p->n[i]);
   }
   p->linv_elems = (size_t)linv;
+  p->med_first = med_first;
+  p->nmed = (int)items.size() - med_first;
+  p->lds_solve = maxn <= kSolveLdsMaxN;
   std::vector<StepWork> sw;
   std::vector<TileWork> tw;
   p->steps.clear();

@@ -240,6 +240,75 @@ hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t ld
   return hipSuccess;
 }
 
+namespace {
+
+// Half of m rounded to whole kNB blocks (at least one block).
+inline int split_blocks(int m) { return ((m / kNB + 1) / 2) * kNB; }
+
+// W := L^-1 for the m x m lower-triangular block of L at (o, o) (ld lda), into
+// W at (o, o) (ld ldw), by halves: [[L11, 0], [L21, L22]]^-1 =
+// [[W11, 0], [-W22 L21 W11, W22]].  A kNB block copies its inverse from dinv
+// (inv(L_kk) of every diagonal block, formed by the factor).  tmp holds the
+// n2 x n1 product L21 W11.
+template <typename T>
+hipError_t trtri_rec(int o, int m, const T* L, int64_t lda, const T* dinv, T* W, int64_t ldw, T* tmp,
+                     hipStream_t s) {
+  if (m <= kNB)
+    return launch_copy_lower<T>(m, dinv + (int64_t)(o / kNB) * kNB * kNB, kNB, W + (int64_t)o * ldw + o, ldw, s);
+  const int n1 = split_blocks(m), n2 = m - n1;
+  LLH_TRY(trtri_rec<T>(o, n1, L, lda, dinv, W, ldw, tmp, s));
+  LLH_TRY(trtri_rec<T>(o + n1, n2, L, lda, dinv, W, ldw, tmp, s));
+  const T* l21 = L + (int64_t)(o + n1) * lda + o;
+  const T* w11 = W + (int64_t)o * ldw + o;
+  const T* w22 = W + (int64_t)(o + n1) * ldw + o + n1;
+  T* w21 = W + (int64_t)(o + n1) * ldw + o;
+  LLH_TRY(launch_gemm<T>(false, false, false, n2, n1, n1, T(1), l21, lda, w11, ldw, T(0), tmp, n1, nullptr, s));
+  return launch_gemm<T>(false, false, false, n2, n1, n2, T(-1), w22, ldw, tmp, n1, T(0), w21, ldw, nullptr, s);
+}
+
+// X := the lower triangle of W^T W for the m x m lower-triangular block of W at
+// (o, o), by halves: with W = [[W11, 0], [W21, W22]],
+// W^T W = [[W11^T W11 + W21^T W21, .], [W22^T W21, W22^T W22]].  (The blocks of
+// X above the diagonal of each product are written too and then overwritten
+// by the mirror.)
+template <typename T>
+hipError_t lauum_rec(int o, int m, const T* W, int64_t ldw, T* X, int64_t ldx, hipStream_t s) {
+  const T* w = W + (int64_t)o * ldw + o;
+  T* x = X + (int64_t)o * ldx + o;
+  if (m <= kNB) return launch_gemm<T>(true, false, false, m, m, m, T(1), w, ldw, w, ldw, T(0), x, ldx, nullptr, s);
+  const int n1 = split_blocks(m), n2 = m - n1;
+  const T* w21 = W + (int64_t)(o + n1) * ldw + o;
+  const T* w22 = W + (int64_t)(o + n1) * ldw + o + n1;
+  T* x21 = X + (int64_t)(o + n1) * ldx + o;
+  LLH_TRY(lauum_rec<T>(o, n1, W, ldw, X, ldx, s));
+  LLH_TRY(launch_gemm<T>(true, false, false, n1, n1, n2, T(1), w21, ldw, w21, ldw, T(1), x, ldx, nullptr, s));
+  LLH_TRY(launch_gemm<T>(true, false, false, n2, n1, n2, T(1), w22, ldw, w21, ldw, T(0), x21, ldx, nullptr, s));
+  return lauum_rec<T>(o + n1, n2, W, ldw, X, ldx, s);
+}
+
+}  // namespace
+
+// A^-1 = L^-T L^-1 from the factor L (the lower triangle of a, lda) and the
+// inverses of its diagonal blocks: W = L^-1 into w (n x n, ldw; its strict
+// upper triangle is zeroed), then the lower triangle of W^T W into x (ldx),
+// mirrored into the upper triangle.  tmp: (n / 2 + kNB)^2 elements.  Work:
+// about 2n^3/3 flops for W and n^3/2 for W^T W as MFMA products, against the
+// 2n^3 of solving with the n columns of the identity.
+template <typename T>
+hipError_t potri_enqueue(int n, const T* a, int64_t lda, const T* dinv, T* w, int64_t ldw, T* x, int64_t ldx,
+                         T* tmp, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  LLH_TRY(launch_zero_triangle<T>(w, ldw, n, true, nullptr, s));
+  LLH_TRY(trtri_rec<T>(0, n, a, lda, dinv, w, ldw, tmp, s));
+  LLH_TRY(lauum_rec<T>(0, n, w, ldw, x, ldx, s));
+  return launch_mirror_lower<T>(x, ldx, n, s);
+}
+
+template hipError_t potri_enqueue<double>(int, const double*, int64_t, const double*, double*, int64_t, double*,
+                                          int64_t, double*, hipStream_t);
+template hipError_t potri_enqueue<float>(int, const float*, int64_t, const float*, float*, int64_t, float*, int64_t,
+                                         float*, hipStream_t);
+
 template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool, double*, hipStream_t,
                                           Profiler*, bool, const Lookahead*, _Float16*);
 template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, float*, hipStream_t, Profiler*,

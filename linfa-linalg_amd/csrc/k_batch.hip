@@ -179,6 +179,182 @@ __global__ __launch_bounds__(256) void grp_bwd_kernel(const BatchBase<T>* __rest
   }
 }
 
+// Forward and back substitution of one system per workgroup, all of it in one
+// launch (n <= kSolveLdsMaxN): the system's right-hand sides (8 columns at a
+// time) stay in LDS from the first block to the last, and L streams through a
+// 64 x 64 LDS tile that is loaded with whole rows per wave (coalesced) while the
+// previous tile is being used (register prefetch).
+//   forward, block k:  Y_k = inv(L_kk) B_k, then B_i -= L_ik Y_k for the
+//                      64-row chunks i below (L read along its rows);
+//   backward, block k: X_k = inv(L_kk)^T Y_k, then B_i -= L_ki^T X_k for the
+//                      64-column chunks i left of k (L's block row k, read
+//                      along its rows, the transpose taken in LDS).
+// Thread (r, c) owns row (or column) r of the 64-block and columns c, c + 4.
+template <typename T>
+__global__ __launch_bounds__(256) void grp_solve_kernel(const BatchBase<T>* __restrict__ base,
+                                                        const BatchItem* __restrict__ items, int count) {
+  constexpr int MC = 8, S = kNB + 1;
+  if ((int)blockIdx.x >= count) return;
+  const BatchItem it = items[blockIdx.x];
+  const BatchBase<T> bb = *base;
+  if (failed(bb, it)) return;
+  __shared__ T Bs[kSolveLdsMaxN * MC];
+  __shared__ T Ts[kNB * S];
+  const int tid = threadIdx.x, n = it.n, m = it.m, nb = (n + kNB - 1) / kNB;
+  const int64_t lda = it.lda;
+  const T* a = bb.a + it.a_off;
+  const T* linv = bb.linv + it.linv_off;
+  T* bm = bb.b + it.b_off;
+  const int r = tid >> 2, c = tid & 3;
+  T pre[kNB * kNB / 256];
+  // a tile: rows x cols of src (ld), zero outside; element e = tid + 256 g is
+  // (row e / 64, column e % 64), so each wave reads one row segment
+  auto fetch = [&](const T* src, int64_t ld, int rows, int cols) {
+#pragma unroll
+    for (int g = 0; g < kNB * kNB / 256; ++g) {
+      const int e = tid + 256 * g, i = e >> 6, j = e & 63;
+      pre[g] = (i < rows && j < cols) ? src[(int64_t)i * ld + j] : T(0);
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int g = 0; g < kNB * kNB / 256; ++g) {
+      const int e = tid + 256 * g;
+      Ts[(e >> 6) * S + (e & 63)] = pre[g];
+    }
+  };
+  for (int c0 = 0; c0 < m; c0 += MC) {
+    const int mc = min(MC, m - c0);
+    __syncthreads();
+    for (int e = tid; e < nb * kNB * MC; e += 256) {
+      const int row = e / MC, cc = e % MC;
+      Bs[e] = (row < n && cc < mc) ? bm[(int64_t)row * m + c0 + cc] : T(0);
+    }
+    // forward: tile (kb, i0), i0 < 0 the inverse of block kb, else rows i0.. below it
+    {
+      auto src = [&](int kb, int i0) {
+        const int k0 = kb * kNB;
+        if (i0 < 0) fetch(linv + (int64_t)kb * kNB * kNB, kNB, kNB, kNB);
+        else fetch(a + (int64_t)i0 * lda + k0, lda, min(kNB, n - i0), min(kNB, n - k0));
+      };
+      auto advance = [&](int& kb, int& i0) -> bool {
+        const int k0 = kb * kNB, nx = i0 < 0 ? k0 + min(kNB, n - k0) : i0 + kNB;
+        if (nx < n) {
+          i0 = nx;
+          return true;
+        }
+        if (kb + 1 < nb) {
+          kb += 1;
+          i0 = -1;
+          return true;
+        }
+        return false;
+      };
+      int kb = 0, i0 = -1;
+      src(kb, i0);
+      for (;;) {
+        __syncthreads();
+        stash();
+        __syncthreads();
+        int nkb = kb, ni0 = i0;
+        const bool more = advance(nkb, ni0);
+        if (more) src(nkb, ni0);
+        const int k0 = kb * kNB, b = min(kNB, n - k0);
+        const T* yk = Bs + (int64_t)k0 * MC;
+        T s0 = T(0), s1 = T(0);
+        if (i0 < 0) {
+          for (int k = 0; k <= r && k < b; ++k) {
+            const T l = Ts[r * S + k];
+            s0 += l * yk[k * MC + c];
+            s1 += l * yk[k * MC + c + 4];
+          }
+          __syncthreads();
+          if (r < b) {
+            Bs[(k0 + r) * MC + c] = s0;
+            Bs[(k0 + r) * MC + c + 4] = s1;
+          }
+        } else {
+          for (int k = 0; k < b; ++k) {
+            const T l = Ts[r * S + k];
+            s0 += l * yk[k * MC + c];
+            s1 += l * yk[k * MC + c + 4];
+          }
+          if (i0 + r < n) {
+            Bs[(i0 + r) * MC + c] -= s0;
+            Bs[(i0 + r) * MC + c + 4] -= s1;
+          }
+        }
+        if (!more) break;
+        kb = nkb;
+        i0 = ni0;
+      }
+    }
+    // backward: tile (kb, i0), i0 < 0 the inverse of block kb, else columns
+    // i0 .. i0 + 63 of L's block row kb
+    {
+      auto src = [&](int kb, int i0) {
+        const int k0 = kb * kNB;
+        if (i0 < 0) fetch(linv + (int64_t)kb * kNB * kNB, kNB, kNB, kNB);
+        else fetch(a + (int64_t)k0 * lda + i0, lda, min(kNB, n - k0), kNB);
+      };
+      auto advance = [&](int& kb, int& i0) -> bool {
+        const int k0 = kb * kNB, nx = i0 < 0 ? 0 : i0 + kNB;
+        if (nx < k0) {
+          i0 = nx;
+          return true;
+        }
+        if (kb > 0) {
+          kb -= 1;
+          i0 = -1;
+          return true;
+        }
+        return false;
+      };
+      int kb = nb - 1, i0 = -1;
+      src(kb, i0);
+      for (;;) {
+        __syncthreads();
+        stash();
+        __syncthreads();
+        int nkb = kb, ni0 = i0;
+        const bool more = advance(nkb, ni0);
+        if (more) src(nkb, ni0);
+        const int k0 = kb * kNB, b = min(kNB, n - k0);
+        const T* xk = Bs + (int64_t)k0 * MC;
+        T s0 = T(0), s1 = T(0);
+        if (i0 < 0) {
+          for (int k = r; k < b; ++k) {
+            const T l = Ts[k * S + r];
+            s0 += l * xk[k * MC + c];
+            s1 += l * xk[k * MC + c + 4];
+          }
+          __syncthreads();
+          if (r < b) {
+            Bs[(k0 + r) * MC + c] = s0;
+            Bs[(k0 + r) * MC + c + 4] = s1;
+          }
+        } else {
+          for (int k = 0; k < b; ++k) {
+            const T l = Ts[k * S + r];
+            s0 += l * xk[k * MC + c];
+            s1 += l * xk[k * MC + c + 4];
+          }
+          Bs[(i0 + r) * MC + c] -= s0;
+          Bs[(i0 + r) * MC + c + 4] -= s1;
+        }
+        if (!more) break;
+        kb = nkb;
+        i0 = ni0;
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < n * MC; e += 256) {
+      const int row = e / MC, cc = e % MC;
+      if (cc < mc) bm[(int64_t)row * m + c0 + cc] = Bs[e];
+    }
+  }
+}
+
 // Elementwise steps of the mixed-precision batch (pointers read from the device
 // base, so a replayed graph follows the caller's buffers).
 __global__ void mixed_flat_kernel(const MixedBase* __restrict__ base, int op, int64_t elems) {
@@ -255,6 +431,15 @@ LLH_GRP_LAUNCH(launch_grp_panel, grp_panel_kernel, TileWork)
 LLH_GRP_LAUNCH(launch_grp_syrk, grp_syrk_kernel, TileWork)
 LLH_GRP_LAUNCH(launch_grp_fwd, grp_fwd_kernel, StepWork)
 LLH_GRP_LAUNCH(launch_grp_bwd, grp_bwd_kernel, StepWork)
+
+template <typename T>
+hipError_t launch_grp_solve(const BatchBase<T>* base, const BatchItem* items, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL((grp_solve_kernel<T>), dim3(count), dim3(256), 0, s, base, items, count);
+  return hipGetLastError();
+}
+template hipError_t launch_grp_solve<double>(const BatchBase<double>*, const BatchItem*, int, hipStream_t);
+template hipError_t launch_grp_solve<float>(const BatchBase<float>*, const BatchItem*, int, hipStream_t);
 
 hipError_t launch_mixed_flat(const MixedBase* mb, int op, int64_t elems, hipStream_t s) {
   if (elems <= 0) return hipSuccess;

@@ -316,7 +316,11 @@ hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha,
       if (tri) return launch_nt<T, 64, 64, 2, 2, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
       return launch_nt<T, 64, 64, 2, 2, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
     }
-    const bool big = (int64_t)M * N >= (int64_t)4096 * 4096;
+    // 256 x 128 only for f64: the f32 instantiation of that tile ran the
+    // look-ahead panel update (M = 31744, N = K = 1024) at 5.3 ms per launch
+    // against 2.0 ms for f64 on the same shapes, while 128 x 128 is the faster
+    // f32 tile for the SYRK too.
+    const bool big = sizeof(T) == 8 && (int64_t)M * N >= (int64_t)4096 * 4096;
     if (tri) {
       if (big) return launch_nt<T, 256, 128, 4, 2, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
       return launch_nt<T, 128, 128, 2, 2, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);

@@ -64,4 +64,13 @@ hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t ld
                         const T* dinv, bool dinv_t, T* dinv_work, SolveSync* sync, hipStream_t s,
                         Profiler* prof = nullptr, const int32_t* gate = nullptr);
 
+// Enqueues A^-1 = L^-T L^-1 from the factor L (lower triangle of a) and the
+// inverses of its 64 x 64 diagonal blocks (dinv, as potrf_enqueue leaves them):
+// W = L^-1 into w, then W^T W into x (both triangles).  tmp holds
+// potri_tmp_elems(n) elements.
+template <typename T>
+hipError_t potri_enqueue(int n, const T* a, int64_t lda, const T* dinv, T* w, int64_t ldw, T* x, int64_t ldx,
+                         T* tmp, hipStream_t s);
+inline int64_t potri_tmp_elems(int n) { return (int64_t)(n / 2 + kNB) * (n / 2 + kNB); }
+
 }  // namespace llh

@@ -109,6 +109,11 @@ template <typename T>
 hipError_t launch_grp_bwd(const BatchBase<T>* base, const BatchItem* items, const StepWork* work, int count,
                           hipStream_t s);
 hipError_t launch_fill_status(int32_t* status, int64_t count, hipStream_t s);
+// k_batch.hip: forward and back substitution of items[0..count), one workgroup
+// per system, right-hand sides in LDS; every n <= kSolveLdsMaxN.
+constexpr int kSolveLdsMaxN = 512;
+template <typename T>
+hipError_t launch_grp_solve(const BatchBase<T>* base, const BatchItem* items, int count, hipStream_t s);
 
 // k_diag.hip: fast factor of one b x b (b <= 64) diagonal block in place, and
 // its inverse into inv (64 x 64 row-major); a failure at block row f sets
@@ -222,5 +227,11 @@ template <typename T>
 hipError_t launch_is_triangular(const T* a, int64_t lda, int n, bool upper, int32_t* flag, hipStream_t s);
 template <typename T>
 hipError_t launch_set_identity(T* a, int64_t lda, int n, hipStream_t s);
+// a's strict upper triangle := the transpose of its strict lower triangle.
+template <typename T>
+hipError_t launch_mirror_lower(T* a, int64_t lda, int n, hipStream_t s);
+// dst (m x m) := the lower triangle of src with zeros above it.
+template <typename T>
+hipError_t launch_copy_lower(int m, const T* src, int64_t lds, T* dst, int64_t ldd, hipStream_t s);
 
 }  // namespace llh

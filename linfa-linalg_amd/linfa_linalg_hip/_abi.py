@@ -142,6 +142,7 @@ def _declare(lib):
         f("dev_potrf").argtypes = [_vp, _sz, _vp, _sz, _int, _err]
         f("dev_solve_triangular").argtypes = [_vp, _int, _int, _sz, _vp, _sz, _sz, _vp, _sz, _err]
         f("dev_solvec").argtypes = [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _err]
+        f("dev_invc").argtypes = [_vp, _sz, _vp, _sz, _vp, _sz, _err]
         f("dev_orthonormalize").argtypes = [_vp, _sz, _sz, _vp, _sz, _vp, _sz, _err]
         f("dev_apply_constraints").argtypes = [_vp, _sz, _sz, _vp, _sz, _sz, _vp, _sz, _vp, _sz, _err]
     lib.llh_dev_gen_matrix_d.argtypes = [_vp, _int, ctypes.c_uint64, ctypes.c_uint64, _sz, _sz, _vp, _sz, _err]

@@ -114,6 +114,18 @@ def solvec(a, b):
     return b
 
 
+def invc(a, out):
+    """invc_inplace on device tensors: a := its dirty factor, out := a^-1
+    (both triangles)."""
+    _check(a)
+    _check(out)
+    n = a.shape[0]
+    if a.shape[1] != n or tuple(out.shape) != (n, n):
+        raise ValueError("square matrices of one size expected")
+    _call("invc", a, n, ctypes.c_void_p(a.data_ptr()), a.stride(0), ctypes.c_void_p(out.data_ptr()), out.stride(0))
+    return out
+
+
 def orthonormalize(v):
     """LOBPCG's orthonormalize (lobpcg/algorithm.rs:75-97): returns (u, r) with
     r = cholesky(V^T V) (lower) and u = V r^-T; u is v, overwritten in place."""

@@ -117,6 +117,18 @@ impl<A: HipScalar> DeviceMatrix<A> {
     }
 }
 
+impl<A: HipScalar> DeviceMatrix<A> {
+    /// invc_inplace on device memory: self := its dirty factor, the result is
+    /// A^-1 (both triangles; L^-T L^-1 on MFMA above the exact kernel's bound).
+    pub fn invc(&mut self) -> Result<DeviceMatrix<A>> {
+        let n = self.square()?;
+        let out = DeviceMatrix::<A>::uninit(n, n);
+        let (a, o) = (self.ptr, out.ptr);
+        call(|c, e| unsafe { A::dev_invc(c, n, a, n, o, n, e) })?;
+        Ok(out)
+    }
+}
+
 /// Factor precision of the mixed-precision solve.
 #[derive(Debug, Clone, Copy, PartialEq, Eq)]
 pub enum Precision {

@@ -158,7 +158,8 @@ type Err = *mut llh_error;
 
 macro_rules! plain_and_dev {
     ($t:ty, $chol_dirty:ident, $chol:ident, $tri:ident, $is_tri:ident, $solve_tri:ident, $solve_sys:ident,
-     $solvec:ident, $invc:ident, $dpotrf:ident, $dsolve_tri:ident, $dsolvec:ident, $dortho:ident, $dapply:ident) => {
+     $solvec:ident, $invc:ident, $dpotrf:ident, $dsolve_tri:ident, $dsolvec:ident, $dinvc:ident, $dortho:ident,
+     $dapply:ident) => {
         extern "C" {
             pub fn $chol_dirty(ctx: Ctx, rows: usize, cols: usize, a: *mut $t, rs: isize, cs: isize,
                                err: Err) -> llh_status;
@@ -182,6 +183,8 @@ macro_rules! plain_and_dev {
                                b: *mut $t, ldb: usize, err: Err) -> llh_status;
             pub fn $dsolvec(ctx: Ctx, n: usize, a: *mut $t, lda: usize, m: usize, b: *mut $t, ldb: usize, err: Err)
                             -> llh_status;
+            pub fn $dinvc(ctx: Ctx, n: usize, a: *mut $t, lda: usize, out: *mut $t, ldo: usize, err: Err)
+                          -> llh_status;
             pub fn $dortho(ctx: Ctx, n: usize, k: usize, v: *mut $t, ldv: usize, r: *mut $t, ldr: usize, err: Err)
                            -> llh_status;
             pub fn $dapply(ctx: Ctx, n: usize, k: usize, v: *mut $t, ldv: usize, ky: usize, y: *const $t,
@@ -193,11 +196,11 @@ macro_rules! plain_and_dev {
 plain_and_dev!(f64, llh_cholesky_inplace_dirty_d, llh_cholesky_inplace_d, llh_triangular_inplace_d,
                llh_is_triangular_d, llh_solve_triangular_inplace_d, llh_solve_triangular_system_d,
                llh_solvec_inplace_d, llh_invc_inplace_d, llh_dev_potrf_d, llh_dev_solve_triangular_d,
-               llh_dev_solvec_d, llh_dev_orthonormalize_d, llh_dev_apply_constraints_d);
+               llh_dev_solvec_d, llh_dev_invc_d, llh_dev_orthonormalize_d, llh_dev_apply_constraints_d);
 plain_and_dev!(f32, llh_cholesky_inplace_dirty_s, llh_cholesky_inplace_s, llh_triangular_inplace_s,
                llh_is_triangular_s, llh_solve_triangular_inplace_s, llh_solve_triangular_system_s,
                llh_solvec_inplace_s, llh_invc_inplace_s, llh_dev_potrf_s, llh_dev_solve_triangular_s,
-               llh_dev_solvec_s, llh_dev_orthonormalize_s, llh_dev_apply_constraints_s);
+               llh_dev_solvec_s, llh_dev_invc_s, llh_dev_orthonormalize_s, llh_dev_apply_constraints_s);
 
 extern "C" {
     pub fn llh_context_create(device: c_int, out: *mut Ctx, err: Err) -> llh_status;
@@ -210,6 +213,9 @@ extern "C" {
     pub fn llh_context_kernel_stats(ctx: Ctx, kernel_class: c_int, out: *mut llh_kernel_stats, err: Err)
                                     -> llh_status;
     pub fn llh_context_set_solve_spin_limit(ctx: Ctx, limit: i64, err: Err) -> llh_status;
+
+    pub fn llh_dev_solvec_noalloc_d(ctx: Ctx, n: usize, a: *mut f64, lda: usize, m: usize, b: *mut f64, ldb: usize,
+                                    err: Err) -> llh_status;
 
     pub fn llh_dev_gen_matrix_d(ctx: Ctx, dist: c_int, seed: u64, stream: u64, rows: usize, cols: usize,
                                 x: *mut f64, ldx: usize, err: Err) -> llh_status;

@@ -41,11 +41,13 @@ pub trait HipScalar: NdFloat + sealed::Sealed {
                                    lda: usize, m: usize, b: *mut Self, ldb: usize, e: *mut Err) -> llh_status;
     unsafe fn dev_solvec(c: *mut Ctx, n: usize, a: *mut Self, lda: usize, m: usize, b: *mut Self, ldb: usize,
                          e: *mut Err) -> llh_status;
+    unsafe fn dev_invc(c: *mut Ctx, n: usize, a: *mut Self, lda: usize, out: *mut Self, ldo: usize, e: *mut Err)
+                       -> llh_status;
 }
 
 macro_rules! hip_scalar {
     ($t:ty, $cd:ident, $c:ident, $tri:ident, $istri:ident, $st:ident, $ss:ident, $sv:ident, $inv:ident,
-     $dp:ident, $dst:ident, $dsv:ident) => {
+     $dp:ident, $dst:ident, $dsv:ident, $dinv:ident) => {
         impl HipScalar for $t {
             unsafe fn cholesky_inplace_dirty(c: *mut Ctx, rows: usize, cols: usize, a: *mut Self, rs: isize,
                                              cs: isize, e: *mut Err) -> llh_status {
@@ -96,6 +98,10 @@ macro_rules! hip_scalar {
                                  ldb: usize, e: *mut Err) -> llh_status {
                 ffi::$dsv(c, n, a, lda, m, b, ldb, e)
             }
+            unsafe fn dev_invc(c: *mut Ctx, n: usize, a: *mut Self, lda: usize, out: *mut Self, ldo: usize,
+                               e: *mut Err) -> llh_status {
+                ffi::$dinv(c, n, a, lda, out, ldo, e)
+            }
         }
     };
 }
@@ -103,8 +109,8 @@ macro_rules! hip_scalar {
 hip_scalar!(f64, llh_cholesky_inplace_dirty_d, llh_cholesky_inplace_d, llh_triangular_inplace_d,
             llh_is_triangular_d, llh_solve_triangular_inplace_d, llh_solve_triangular_system_d,
             llh_solvec_inplace_d, llh_invc_inplace_d, llh_dev_potrf_d, llh_dev_solve_triangular_d,
-            llh_dev_solvec_d);
+            llh_dev_solvec_d, llh_dev_invc_d);
 hip_scalar!(f32, llh_cholesky_inplace_dirty_s, llh_cholesky_inplace_s, llh_triangular_inplace_s,
             llh_is_triangular_s, llh_solve_triangular_inplace_s, llh_solve_triangular_system_s,
             llh_solvec_inplace_s, llh_invc_inplace_s, llh_dev_potrf_s, llh_dev_solve_triangular_s,
-            llh_dev_solvec_s);
+            llh_dev_solvec_s, llh_dev_invc_s);

@@ -84,6 +84,28 @@ def test_batch_against_oracle(dtype, small_limit):
                 assert np.abs(x - x_ref).max() <= 1e3 * eps * max(1.0, np.abs(x_ref).max()), n
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_lds_substitution_against_oracle(dtype):
+    """Every blocked system within 512 rows: the whole forward and back
+    substitution of each runs in one workgroup with its right-hand sides in
+    LDS (8 columns at a time: m = 9..20 take several chunks)."""
+    from linfa_linalg_hip import device as dev
+    rng = np.random.default_rng(11)
+    sizes = [129, 130, 191, 192, 193, 255, 256, 257, 400, 447, 448, 449, 511, 512] + \
+        list(rng.integers(129, 513, 30))
+    rhs = [0, 1, 2, 3, 5, 7, 8, 9, 12, 16, 17, 20, 1, 8] + [int(x) for x in rng.integers(1, 9, 30)]
+    mats, bs = make_batch(sizes, rhs, dtype, 17)
+    plan = dev.BatchPlan(sizes, rhs, prec="f64" if dtype == np.float64 else "f32", small_limit=128)
+    fac, xs, st = run(plan, mats, bs, dtype)
+    assert np.all(st == -1)
+    eps = np.finfo(dtype).eps
+    for am, bm, f, x in zip(mats, bs, fac, xs):
+        n = am.shape[0]
+        x_ref = orc.solvec_inplace(np.array(am), np.array(bm))
+        if bm.size:
+            assert np.abs(x - x_ref).max() <= 1e3 * eps * max(1.0, np.abs(x_ref).max()), (n, bm.shape)
+
+
 def test_batch_failure_is_isolated_and_replay():
     from linfa_linalg_hip import device as dev
     sizes = [50, 300, 90, 400, 20]

@@ -134,6 +134,64 @@ def test_invc(n):
     assert np.abs(inv - inv_ref).max() <= 1e-12 * np.abs(inv_ref).max()
 
 
+@pytest.mark.parametrize("n", [129, 177, 640, 1537])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_invc_mfma_inverse(n, dtype):
+    """Above the exact kernel's bound, invc is L^-T L^-1 on MFMA (W = L^-1 by
+    halves, then W^T W): against the oracle's solve with the identity
+    (cholesky.rs:178-182), to the inverse's condition-scaled accuracy; A keeps
+    the dirty factor, as in the reference."""
+    a = spd(n, 7 * n).astype(dtype)
+    fac = np.array(a)
+    inv = llh.invc_inplace(fac)
+    fac_ref = np.array(a)
+    inv_ref = orc.invc_inplace(fac_ref)
+    tol = 1e-12 if dtype == np.float64 else 2e-4
+    assert np.abs(inv - inv_ref).max() <= tol * np.abs(inv_ref).max(), n
+    if n > (128 if dtype == np.float64 else 176):  # the MFMA inverse mirrors its lower triangle
+        assert np.array_equal(inv, inv.T)
+    assert np.abs(np.tril(fac) - np.tril(fac_ref)).max() <= tol * np.abs(fac_ref).max()
+    assert np.array_equal(np.triu(fac, 1), np.triu(np.array(a), 1))  # dirty: upper untouched
+    if dtype == np.float64:
+        assert np.abs(a @ inv - np.eye(n)).max() <= 1e-12
+
+
+def test_invc_strided_and_not_pd():
+    n = 300
+    a = spd(n, 11)
+    big = np.zeros((2 * n, 2 * n))
+    view = big[::2, ::-2]  # strided, inverted columns
+    view[...] = a
+    out = llh.invc(view)
+    assert np.abs(out - orc.invc(a)).max() <= 1e-12 * np.abs(out).max()
+    bad = np.array(a)
+    bad[200, 200] = -1.0
+    with pytest.raises(llh.NotPositiveDefinite):
+        llh.invc(bad)
+
+
+@pytest.mark.parametrize("n", [100, 2000])
+def test_dev_invc(n):
+    import torch
+    from linfa_linalg_hip import device as dev
+    a = spd(n, 13 + n)
+    ta = torch.from_numpy(a).cuda()
+    out = torch.full((n, n), float("nan"), dtype=torch.float64, device="cuda")
+    dev.invc(ta, out)
+    inv_ref = orc.invc(np.array(a))
+    inv = out.cpu().numpy()
+    if n <= 128:  # the exact-order kernel: bit for bit
+        assert np.array_equal(inv, inv_ref)
+    assert np.abs(inv - inv_ref).max() <= 1e-12 * np.abs(inv_ref).max()
+    bad = np.array(a)
+    bad[n // 2, n // 2] = -1.0
+    out2 = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    with pytest.raises(llh.NotPositiveDefinite):
+        dev.invc(torch.from_numpy(bad).cuda(), out2)
+    if n > 128:  # the MFMA inverse leaves out untouched after NotPositiveDefinite
+        assert float(out2.abs().max()) == 0.0
+
+
 # --- error paths ------------------------------------------------------------------
 
 # Blocked orders put the failing column at each position of the 64-column
