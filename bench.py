@@ -72,6 +72,30 @@ def cpu_baseline(sizes, m, seed, big_n):
             "extrapolation": "c*flops fit to the largest sample (not measured)"}
 
 
+def oracle_fixture(n, m, lam, seed):
+    """The multi-threaded oracle's solution of the seeded workload (n, m, lam,
+    seed), as committed under tests/golden/full_size (a column subset), or None."""
+    import numpy as np
+    fix = os.path.join(ROOT, "tests", "golden", "full_size")
+    try:
+        with open(os.path.join(fix, "meta.json")) as f:
+            meta = json.load(f)
+    except OSError:
+        return None
+    for cfg in meta.values():
+        if cfg["n"] == n and cfg["m"] == m and cfg["seed"] == seed and cfg["lam"] == lam:
+            return np.load(os.path.join(fix, cfg["x_ref"]))
+    return None
+
+
+def sol_err(x, x_ref):
+    """||x - x_ref|| / ||x_ref|| over the fixture's columns."""
+    import numpy as np
+    k = x_ref.shape[1]
+    xs = x[:, :k].double().cpu().numpy()
+    return float(np.linalg.norm(xs - x_ref) / np.linalg.norm(x_ref))
+
+
 def batched_leg(dev, torch, args):
     """20000 SPD systems, n log-uniform in [8, 512], A_i = B_i B_i^T / n + I with
     B_i uniform in [-1, 1], 1-8 right-hand sides ~ N(0, 1): one planned,
@@ -204,6 +228,54 @@ def batched_leg(dev, torch, args):
             "fp32_ir": fp32}
 
 
+def invc_leg(dev, torch, args):
+    """InverseC at n = 8192, f64: the dedicated inverse (llh_dev_invc_d: factor,
+    W = L^-1 and W^T W on MFMA) against the reference's own route, solvec on the
+    identity (llh_dev_solvec_d with n right-hand sides).  GFLOP/s count the
+    reference route's algorithmic work, n^3/3 + 2 n^3, for both."""
+    import numpy as np
+    n = 8192
+    a = dev.gen_spd(dev.GEN_NORMAL, args.seed + 5, n, 1.0)
+    eye = torch.eye(n, dtype=torch.float64, device="cuda")
+    out = torch.empty_like(a)
+    fl = n ** 3 / 3.0 + 2.0 * n ** 3
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    def dedicated():
+        w = a.clone()
+        dev.invc(w, out)
+
+    def via_solve():
+        w, x = a.clone(), eye.clone()
+        dev.solvec(w, x)
+        return x
+
+    t_d = timed(dedicated)
+    x_s = via_solve()
+    t_s = timed(via_solve, reps=2)
+    diff = float((out - x_s).norm() / x_s.norm())
+    res = float((a @ out - eye).abs().max())
+    del a, eye, out, x_s
+    torch.cuda.empty_cache()
+    row = {"workload": f"invc n={n} f64 (A = B B^T/n + I, seeded)", "ms": round(t_d * 1e3, 3),
+           "value": round(fl / t_d / 1e9, 1), "unit": "GFLOP/s (reference route's n^3/3 + 2n^3)",
+           "solvec_identity_ms": round(t_s * 1e3, 3), "speedup": round(t_s / t_d, 2),
+           "diff_vs_solvec_identity": diff, "max_abs_A_inv_minus_I": res}
+    log(f"bench: invc {row}")
+    return row
+
+
 def next_rows_leg(dev, torch, args):
     """LOBPCG's Cholesky consumers and QR's back-substitution (the rows beside the
     solve path): device orthonormalize and apply_constraints on a tall block
@@ -281,6 +353,7 @@ def next_rows_leg(dev, torch, args):
     t0 = time.perf_counter()
     orc.solve_triangular_system(r, np.array(bq), 0, d, True)
     t_cpu = time.perf_counter() - t0
+    out["invc"] = invc_leg(dev, torch, args)
     out["qr_solve"] = {"workload": f"R x = b, n={nq}, {mq} RHS, pivots |diag_i| (llh_solve_triangular_system_d, "
                                    "host arrays through the boundary)",
                        "ms": round(float(np.median(ts)) * 1e3, 3), "cpu_oracle_ms": round(t_cpu * 1e3, 3)}
@@ -306,9 +379,13 @@ def serving_leg(dev, torch, args):
         a.diagonal().add_(1.0)
         pool.append(a)
     torch.cuda.synchronize()
+    plan = ((1000, 2.0), (10000, 0.6), (50000, 0.2))
     out = {"workload": "Poisson solve requests, n log-uniform [16,4096] (A pre-staged in HBM), 1-32 RHS, "
-                       "30% id reuse, 50/50 f64 and FP32+IR", "rates": []}
-    for rate, dur in ((1000, 2.0), (10000, 0.6), (50000, 0.2)):
+                       "30% id reuse, 50/50 f64 and FP32+IR; arrival streams of "
+                       + " / ".join(f"{d:g} s at {r // 1000}k/s" for r, d in plan)
+                       + " (not the configuration's 60 s: above saturation latency grows with the stream's "
+                         "length, so the 10k and 50k figures are for these lengths)", "rates": []}
+    for rate, dur in plan:
         log(f"bench: serving at {rate}/s for {dur} s")
         srv = dev.Server()
         cnt = int(rate * dur)
@@ -459,6 +536,45 @@ def rocprof_summary(trace_dir, fetch_dir, write_dir, n, out_md, out_json, comman
     print(json.dumps(res))
 
 
+def pmc_summary(pmc_dir, out_md, command, title):
+    """Summarises one `rocprofv3 --pmc <SQ/GRBM counters>` run of this benchmark
+    per kernel: each counter summed over the kernel's dispatches, the MFMA work
+    the hardware counted (SQ_INSTS_VALU_MFMA_MOPS_* x 512 flops), and the MFMA
+    busy fraction SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 256 CUs x 4
+    SIMDs), the formula rocprofiler-compute uses for MFMA utilisation."""
+    import csv
+    import glob
+    hits = glob.glob(os.path.join(pmc_dir, "**", "*counter_collection.csv"), recursive=True)
+    if not hits:
+        raise SystemExit(f"no counter_collection.csv under {pmc_dir}")
+    per = {}
+    disp = {}
+    for r in csv.DictReader(open(hits[0])):
+        name = r["Kernel_Name"].replace("void ", "").replace("(anonymous namespace)::", "").split("(")[0]
+        per.setdefault(name, {}).setdefault(r["Counter_Name"], 0.0)
+        per[name][r["Counter_Name"]] += float(r["Counter_Value"])
+        disp.setdefault(name, set()).add(r.get("Dispatch_Id", r.get("Correlation_Id", "")))
+    ctrs = sorted({c for v in per.values() for c in v})
+    rows = sorted(per.items(), key=lambda kv: -kv[1].get("GRBM_GUI_ACTIVE", 0.0))
+    lines = [f"# rocprofv3 counters: {title}", "", "Command (run from the repository root on one MI355X):", "",
+             "```", command, "```", "",
+             "Derived: `mfma_tflop` = (SQ_INSTS_VALU_MFMA_MOPS_F64 + _F32) x 512 flops summed over the "
+             "kernel's dispatches; `mfma_busy` = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 256 CUs x 4 SIMDs).",
+             "", "| kernel | dispatches | " + " | ".join(ctrs) + " | mfma_tflop | mfma_busy |",
+             "|---|---:|" + "---:|" * (len(ctrs) + 2)]
+    for name, v in rows[:16]:
+        short = name if len(name) <= 80 else name[:77] + "..."
+        mops = v.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0) + v.get("SQ_INSTS_VALU_MFMA_MOPS_F32", 0.0)
+        busy = v.get("SQ_VALU_MFMA_BUSY_CYCLES")
+        gui = v.get("GRBM_GUI_ACTIVE")
+        util = f"{busy / (gui * 256 * 4):.3f}" if busy is not None and gui else "-"
+        lines.append(f"| `{short}` | {len(disp[name])} | " + " | ".join(f"{v.get(c, 0.0):.4g}" for c in ctrs)
+                     + f" | {mops * 512 / 1e12:.3f} | {util} |")
+    with open(out_md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def load_traffic(n, m):
     """Per-launch HBM bytes of the trailing SYRK from the committed PMC summary."""
     try:
@@ -491,7 +607,16 @@ def main():
     ap.add_argument("--rocprof-summary", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="summarise rocprofv3 runs of this benchmark into profiles/ and exit")
     ap.add_argument("--rocprof-command", default="")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("PMC_DIR", "OUT_MD", "TITLE"),
+                    help="summarise a rocprofv3 --pmc run of SQ/GRBM counters into OUT_MD and exit")
+    ap.add_argument("--lambdas", default="1,1e-2,1e-4,1e-6",
+                    help="lambdas of the mixed-precision sweep (the first is also --lam's leg)")
+    ap.add_argument("--no-f16", action="store_true")
     args = ap.parse_args()
+
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.rocprof_command, args.pmc_summary[2])
+        return
 
     if args.rocprof_summary:
         rocprof_summary(*args.rocprof_summary, args.n,
@@ -510,13 +635,9 @@ def main():
 
     n, m = args.n, args.nrhs
     torch.cuda.set_device(0)
-    g = torch.Generator(device="cuda").manual_seed(args.seed)
-    log(f"bench: generating A = B B^T/n + {args.lam} I, n={n}, f64")
-    bmat = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
-    a0 = torch.matmul(bmat, bmat.T).div_(n)
-    del bmat
-    a0.diagonal().add_(args.lam)
-    rhs0 = torch.randn(n, m, dtype=torch.float64, device="cuda", generator=g)
+    log(f"bench: generating A = B B^T/n + {args.lam} I, n={n}, f64 (seeded counter generator)")
+    a0 = dev.gen_spd(dev.GEN_NORMAL, args.seed, n, args.lam)
+    rhs0 = dev.gen_matrix(dev.GEN_NORMAL, args.seed, 1, n, m)
     copies = [(a0.clone(), rhs0.clone()) for _ in range(args.warmup + args.steps)]
     torch.cuda.synchronize()
 
@@ -580,7 +701,12 @@ def main():
         check["residual"] = float(res.norm() / (a0.norm() * x.norm()))
         al = torch.tril(a0)
         check["backward_error"] = float(torch.tril(l @ l.T - a0).norm() / al.norm())
-        log(f"bench: residual {check['residual']:.3e} backward error {check['backward_error']:.3e}")
+        ref = oracle_fixture(n, m, args.lam, args.seed)
+        if ref is not None:
+            check["solution_error_vs_oracle"] = sol_err(x, ref)
+            check["oracle"] = "multi-threaded oracle (reference operation order), tests/golden/full_size"
+        log(f"bench: residual {check['residual']:.3e} backward error {check['backward_error']:.3e} "
+            f"vs oracle {check.get('solution_error_vs_oracle')}")
         del l, res, al
 
     # Mixed precision on the same A (A is not modified): FP32-MFMA and
@@ -592,7 +718,12 @@ def main():
     mixed16 = None
     sweep = []
     f64_clean_ms = None
+    copies = copies[-1:]  # the last step's factor and solution stay for the comparisons below
+    torch.cuda.empty_cache()
     if not args.no_mixed:
+        # s0 = B B^T / n: s0 + lambda I is bit for bit the generator's A for
+        # lambda, the input the oracle's full-size fixtures were computed on
+        s0 = dev.gen_spd(dev.GEN_NORMAL, args.seed, n, 0.0)
         x32 = torch.zeros(n, m, dtype=torch.float64, device="cuda")
         x64 = copies[-1][1]
         ca, cb = a0.clone(), rhs0.clone()
@@ -607,6 +738,8 @@ def main():
 
         def mixed_run(prec, lam):
             info = dev.solve_mixed(a0, rhs0, x32, prec=prec, fallback=True)
+            ref = oracle_fixture(n, m, lam, args.seed)
+            err_orc = sol_err(x32, ref) if ref is not None else None
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(args.steps):
@@ -619,7 +752,8 @@ def main():
             del res
             row = {"precision": prec, "lambda": lam, "cond_estimate": round((4.0 + lam) / lam, 1),
                    "ms_per_step": round(msx, 3), "value": round(gx, 2), "unit": "GFLOP/s",
-                   "speedup_vs_f64": round(gx / g64_clean, 3), "refinement": info, "backward_residual": bres}
+                   "speedup_vs_f64": round(gx / g64_clean, 3), "refinement": info, "backward_residual": bres,
+                   "solution_error_vs_oracle": err_orc}
             log(f"bench: {prec}+IR lambda={lam:g} {msx:.2f} ms/step, {gx:.1f} GFLOP/s, x{gx / g64_clean:.2f} vs "
                 f"f64, {info}, backward residual {bres:.2e}")
             return row
@@ -637,35 +771,33 @@ def main():
                                          if tot_s > 0 else None)
             row["kernels_ms_profiled_run"] = {k: round(v[1], 3) for k, v in st.items() if v[0]}
 
-        lam_cur = args.lam
-        for lam in [args.lam] + [v for v in (1e-2, 1e-4, 1e-6) if v != args.lam]:
-            a0.diagonal().add_(lam - lam_cur)
-            lam_cur = lam
+        lams = [float(v) for v in args.lambdas.split(",") if v]
+        for lam in [args.lam] + [v for v in lams if v != args.lam]:
+            a0.copy_(s0)
+            a0.diagonal().add_(lam)  # exactly the generator's A for this lambda
             row = mixed_run("f32", lam)
             if lam == args.lam:
                 breakdown(row, "f32", "f32")
                 mixed = row
             sweep.append(row)
-            if lam >= 1e-2:  # the f16 factor is for cond up to ~1e3
+            if lam >= 1e-2 and not args.no_f16:  # the f16 factor is for cond up to ~1e3
                 row16 = mixed_run("f16", lam)
                 if lam == args.lam:
                     breakdown(row16, "f16", "f16")
                     mixed16 = row16
                 sweep.append(row16)
-        a0.diagonal().add_(args.lam - lam_cur)
-        del x32
+        a0.copy_(s0)
+        a0.diagonal().add_(args.lam)
+        del x32, s0
 
     # C2: n = 16384, 64 right-hand sides, lambda = 0.1, f64
     c2 = None
     if not args.no_c2:
         n2, m2, lam2 = 16384, 64, 0.1
         del copies
-        g2 = torch.Generator(device="cuda").manual_seed(args.seed + 2)
-        bm2 = torch.randn(n2, n2, dtype=torch.float64, device="cuda", generator=g2)
-        a2 = torch.matmul(bm2, bm2.T).div_(n2)
-        del bm2
-        a2.diagonal().add_(lam2)
-        r2 = torch.randn(n2, m2, dtype=torch.float64, device="cuda", generator=g2)
+        torch.cuda.empty_cache()
+        a2 = dev.gen_spd(dev.GEN_NORMAL, args.seed, n2, lam2)
+        r2 = dev.gen_matrix(dev.GEN_NORMAL, args.seed, 1, n2, m2)
         cps = [(a2.clone(), r2.clone()) for _ in range(args.warmup + args.steps)]
         torch.cuda.synchronize()
         for i in range(args.warmup):
@@ -678,11 +810,18 @@ def main():
         ms2 = (time.perf_counter() - t0) * 1e3 / args.steps
         xf = cps[-1][1]
         res = r2 - a2 @ xf
+        ref2 = oracle_fixture(n2, m2, lam2, args.seed)
+        x2 = torch.zeros_like(r2)
+        info2 = dev.solve_mixed(a2, r2, x2, prec="f32", fallback=True)
         c2 = {"workload": f"spd_solve n={n2} nrhs={m2} lambda={lam2} f64 (llh_dev_solvec_d)",
               "ms_per_step": round(ms2, 3), "value": round(solve_flops(n2, m2) / (ms2 / 1e3) / 1e9, 2),
               "unit": "GFLOP/s", "roofline_frac_f64": round(solve_flops(n2, m2) / (ms2 / 1e3) / 1e12 /
                                                            PEAK_TFLOPS["f64"], 4),
-              "residual": float(res.norm() / (a2.norm() * xf.norm()))}
+              "residual": float(res.norm() / (a2.norm() * xf.norm())),
+              "solution_error_vs_oracle": sol_err(xf, ref2) if ref2 is not None else None,
+              "fp32_ir_solution_error_vs_oracle": sol_err(x2, ref2) if ref2 is not None else None,
+              "fp32_ir_refinement": info2}
+        del x2
         log(f"bench: C2 {c2}")
         del cps, a2, r2, res
         torch.cuda.empty_cache()
@@ -748,7 +887,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded N(0,1) B, A = B B^T/n + lambda I, built on device)",
+        "data": "synthetic (seeded counter-based N(0,1) B, A = B B^T/n + lambda I, built on device; "
+                "llh_dev_gen_spd_d, reproduced bit for bit by the oracle's generator)",
         "config": {"workload": f"spd_solve n={n} nrhs={m} lambda={args.lam} f64 (llh_dev_solvec_d)",
                    "n": n, "nrhs": m, "lambda": args.lam, "parallelism": "1 GPU"},
         "solve_roofline_frac_f64": round(gflops / 1e3 / PEAK_TFLOPS["f64"], 4),
