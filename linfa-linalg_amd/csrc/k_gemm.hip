@@ -247,7 +247,10 @@ __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_ker
   // Measured at n = 32256, k = 512: f64 53 -> 57, f16 199 -> 348 TFLOP/s.  Not
   // for f32 operands (88 -> 107 TFLOP/s): folding C into the f32 sum mid-way
   // costs the accuracy that refinement then pays for (11 instead of 3 steps at
-  // cond ~4e6), and holding C to the end does not fit the registers.
+  // cond ~4e6), and holding C to the end does not fit the registers.  Seeding
+  // the f32 accumulators with C before the first slab is no better: 119 instead
+  // of 104 TFLOP/s alone, no gain inside the factor (149.6 -> 148.4 ms at
+  // n = 32768), and 29 instead of 4 refinement steps at lambda = 1e-6.
   constexpr bool PRE = SQ && ((BM == 256 && sizeof(T) == 8) || sizeof(TI) == 2);
   nt_tile<T, BM, BN, WM, WN, TRI, VEC, PRE, TI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
 }
