@@ -107,23 +107,48 @@ def batched_leg(dev, torch, args):
     sizes = np.floor(np.exp(rng.uniform(np.log(8), np.log(512), cnt))).astype(np.int32)
     sizes = np.minimum(sizes, 512)
     rhs = rng.integers(1, 9, cnt).astype(np.int32)
-    plan = dev.BatchPlan(sizes, rhs, prec="f64")
+    once = [int(v) for v in args.batch_once.split(",")] if args.batch_once else None
+    plan = (dev.BatchPlan(sizes, rhs, prec="f64", small_limit=once[0], small_waves=once[1]) if once
+            else dev.BatchPlan(sizes, rhs, prec="f64"))
     a = torch.zeros(plan.a_elems, dtype=torch.float64, device="cuda")
     b = torch.zeros(plan.b_elems, dtype=torch.float64, device="cuda")
     g = torch.Generator(device="cuda").manual_seed(args.seed + 5)
+    # one scatter per size (not one copy per system): the systems of a size
+    # share a leading dimension
     for s in np.unique(sizes):
+        s = int(s)
         idx = np.nonzero(sizes == s)[0]
         bm = torch.rand(len(idx), s, s, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
         am = torch.bmm(bm, bm.transpose(1, 2).contiguous()) / s
-        am += torch.eye(int(s), dtype=torch.float64, device="cuda")
+        am += torch.eye(s, dtype=torch.float64, device="cuda")
         ld = int(plan.lda[idx[0]])
-        for j, i in enumerate(idx):
-            o = int(plan.a_off[i])
-            a[o:o + s * ld].view(s, ld)[:, :s].copy_(am[j])
-            ob, m = int(plan.b_off[i]), int(rhs[i])
-            b[ob:ob + s * m].normal_(generator=g)
+        assert np.all(plan.lda[idx] == ld)
+        within = (np.arange(s)[:, None] * ld + np.arange(s)[None, :]).reshape(-1)
+        flat = (plan.a_off[idx].astype(np.int64)[:, None] + within[None, :]).reshape(-1)
+        a[torch.from_numpy(flat).cuda()] = am.reshape(-1)
+        bflat = np.concatenate([int(o) + np.arange(s * int(m)) for o, m in zip(plan.b_off[idx], rhs[idx])])
+        b[torch.from_numpy(bflat).cuda()] = torch.randn(len(bflat), dtype=torch.float64, device="cuda",
+                                                        generator=g)
     status = torch.zeros(cnt, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
+    n64 = sizes.astype(np.float64)
+    flops = float(np.sum(n64 ** 3 / 3 + n64 ** 2 / 2 + n64 / 6 + 2 * n64 ** 2 * rhs))
+    byts = float(np.sum(8 * n64 * (n64 + 1) + 16 * n64 * rhs))
+    if once:
+        # exactly one graph launch with the given configuration and nothing
+        # else of the library's, for counter runs; the clone of A is a copy of
+        # known size (a_elems x 8 bytes) that scales the HBM counters
+        a_copy = a.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.solve(a, b, status)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        res = {"batch_once": True, "count": cnt, "config": plan.config, "ms_first_launch": round(t * 1e3, 3),
+               "ok": int((status == -1).sum().item()), "a_copy_bytes": int(a_copy.numel()) * 8,
+               "algorithmic_bytes": byts, "algorithmic_flops": flops}
+        plan.close()
+        return res
     log("bench: batched autotune")
     tuned_ms = plan.autotune(a, b, status)
     a0, b0 = a.clone(), b.clone()
@@ -139,9 +164,6 @@ def batched_leg(dev, torch, args):
             times.append(t1 - t0)
     t = float(np.median(times))
     ok = int((status == -1).sum().item())
-    n64 = sizes.astype(np.float64)
-    flops = float(np.sum(n64 ** 3 / 3 + n64 ** 2 / 2 + n64 / 6 + 2 * n64 ** 2 * rhs))
-    byts = float(np.sum(8 * n64 * (n64 + 1) + 16 * n64 * rhs))
     roof_t = max(flops / (PEAK_TFLOPS["f64"] * 1e12), byts / (PEAK_HBM_GBS * 1e9))
     # parity sample against the oracle
     from oracle import oracle as orc
@@ -219,6 +241,7 @@ def batched_leg(dev, torch, args):
             "gbs_algorithmic": round(byts / t / 1e9, 1), "roofline_frac": round(roof_t / t, 4),
             "roofline_bound": "mfma" if flops / (PEAK_TFLOPS["f64"] * 1e12) > byts / (PEAK_HBM_GBS * 1e9)
             else "hbm", "config": cfg64, "autotuned_ms": round(tuned_ms, 3),
+            "traffic": load_batched_traffic(cnt, cfg64),
             "ok": ok, "max_rel_err_vs_oracle_sample": worst,
             "plain_per_matrix": {"solves_per_s_sample": round(plain_rate, 1),
                                  "solves_per_s_same_mix_est": round(plain_rate_scaled, 1) if plain_rate_scaled
@@ -575,6 +598,88 @@ def pmc_summary(pmc_dir, out_md, command, title):
     print("\n".join(lines))
 
 
+BATCHED_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "batched_uneven_20000_traffic.json")
+
+
+def batched_summary(trace_dir, fetch_dir, write_dir, once_json, command):
+    """Summarises three rocprofv3 runs of `bench.py --batch-once` (one graph
+    launch of the batched workload): per-kernel durations (`--kernel-trace`),
+    and HBM bytes per kernel (`--pmc FETCH_SIZE`, then `--pmc WRITE_SIZE`),
+    scaled by what the same counters read for the clone of the packed A (a copy
+    of known size).  The launch's traffic is set against its algorithmic bytes
+    (read and write each lower triangle once, read and write each B once)."""
+    import csv
+    import glob
+
+    def one(d, pat):
+        hits = glob.glob(os.path.join(d, "**", pat), recursive=True)
+        if not hits:
+            raise SystemExit(f"no {pat} under {d}")
+        return hits[0]
+
+    def short(name):
+        s = name.replace("void ", "").replace("(anonymous namespace)::", "").split("(")[0]
+        return s if len(s) <= 70 else s[:67] + "..."
+
+    with open(once_json) as f:
+        once = json.loads([ln for ln in f.read().splitlines() if ln.startswith("{")][-1])
+    known_kib = once["a_copy_bytes"] / 1024.0
+    durs = {}
+    for r in csv.DictReader(open(one(trace_dir, "*kernel_trace.csv"))):
+        if short(r["Kernel_Name"]).startswith("llh::"):
+            durs.setdefault(short(r["Kernel_Name"]), []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    per = {}
+    factors = {}
+    for ctr, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = [r for r in csv.DictReader(open(one(d, "*counter_collection.csv"))) if r["Counter_Name"] == ctr]
+        copy = max(float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"])
+        factors[ctr] = {"copy_kib": copy, "known_kib": known_kib, "factor": known_kib / copy}
+        for r in rows:
+            if short(r["Kernel_Name"]).startswith("llh::"):
+                per.setdefault(short(r["Kernel_Name"]), {}).setdefault(ctr, 0.0)
+                per[short(r["Kernel_Name"])][ctr] += float(r["Counter_Value"]) * 1024 * factors[ctr]["factor"]
+    total = sum(v.get("FETCH_SIZE", 0.0) + v.get("WRITE_SIZE", 0.0) for v in per.values())
+    dev_ns = sum(sum(v) for v in durs.values())
+    res = {"count": once["count"], "config": once["config"], "algorithmic_bytes": once["algorithmic_bytes"],
+           "traffic_bytes": total, "traffic_over_algorithmic": total / once["algorithmic_bytes"],
+           "kernel_time_ms": dev_ns / 1e6, "factors": factors,
+           "kernels": {k: {"dispatches": len(durs.get(k, [])), "ms": sum(durs.get(k, [])) / 1e6,
+                           "fetch_bytes": v.get("FETCH_SIZE", 0.0), "write_bytes": v.get("WRITE_SIZE", 0.0)}
+                       for k, v in per.items()},
+           "command": command}
+    with open(BATCHED_TRAFFIC_FILE, "w") as f:
+        json.dump(res, f, indent=1)
+    lines = [f"# rocprofv3 summary: {once['count']} uneven SPD systems, one hipGraph launch (bench.py --batch-once)",
+             "", "Commands (run from the repository root on one MI355X):", "", "```", command, "```", "",
+             f"Configuration {once['config']}; the launch: {once['ok']} of {once['count']} systems solved. "
+             f"Counter scale factors from the clone of the packed A ({once['a_copy_bytes'] / 2**20:.1f} MiB): "
+             f"FETCH_SIZE x{factors['FETCH_SIZE']['factor']:.3f}, WRITE_SIZE x{factors['WRITE_SIZE']['factor']:.3f}.",
+             "", "| kernel | dispatches | ms | HBM read (MB) | HBM write (MB) | GB/s |", "|---|---:|---:|---:|---:|---:|"]
+    for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["ms"]):
+        gbs = (v["fetch_bytes"] + v["write_bytes"]) / (v["ms"] / 1e3) / 1e9 if v["ms"] > 0 else 0.0
+        lines.append(f"| `{k}` | {v['dispatches']} | {v['ms']:.3f} | {v['fetch_bytes'] / 1e6:.1f} | "
+                     f"{v['write_bytes'] / 1e6:.1f} | {gbs:.0f} |")
+    lines += ["", f"Kernel time {dev_ns / 1e6:.3f} ms in all. HBM traffic {total / 1e9:.3f} GB against "
+                  f"{once['algorithmic_bytes'] / 1e9:.3f} GB algorithmic: {total / once['algorithmic_bytes']:.2f}x.", ""]
+    with open(os.path.join(ROOT, "profiles", "batched_uneven_20000.md"), "w") as f:
+        f.write("\n".join(lines))
+    print("\n".join(lines))
+
+
+def load_batched_traffic(cnt, config):
+    """The batched launch's HBM traffic from the committed summary, when it was
+    measured on this workload and configuration."""
+    try:
+        with open(BATCHED_TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("count") != cnt or t.get("config") != config:
+        return None
+    return {"bytes": round(t["traffic_bytes"]), "over_algorithmic": round(t["traffic_over_algorithmic"], 3),
+            "source": "profiles/batched_uneven_20000.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, one launch)"}
+
+
 def load_traffic(n, m):
     """Per-launch HBM bytes of the trailing SYRK from the committed PMC summary."""
     try:
@@ -612,7 +717,16 @@ def main():
     ap.add_argument("--lambdas", default="1,1e-2,1e-4,1e-6",
                     help="lambdas of the mixed-precision sweep (the first is also --lam's leg)")
     ap.add_argument("--no-f16", action="store_true")
+    ap.add_argument("--batch-once", default="", metavar="SMALL_LIMIT,SMALL_WAVES",
+                    help="run one f64 graph launch of the batched workload with this configuration, print "
+                         "its JSON and exit (for counter runs)")
+    ap.add_argument("--batched-summary", nargs=4, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR", "ONCE_JSON"),
+                    help="summarise rocprofv3 runs of --batch-once into profiles/ and exit")
     args = ap.parse_args()
+
+    if args.batched_summary:
+        batched_summary(*args.batched_summary, args.rocprof_command)
+        return
 
     if args.pmc_summary:
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.rocprof_command, args.pmc_summary[2])
@@ -635,6 +749,9 @@ def main():
 
     n, m = args.n, args.nrhs
     torch.cuda.set_device(0)
+    if args.batch_once:
+        print(json.dumps(batched_leg(dev, torch, args)))
+        return
     log(f"bench: generating A = B B^T/n + {args.lam} I, n={n}, f64 (seeded counter generator)")
     a0 = dev.gen_spd(dev.GEN_NORMAL, args.seed, n, args.lam)
     rhs0 = dev.gen_matrix(dev.GEN_NORMAL, args.seed, 1, n, m)
