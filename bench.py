@@ -563,8 +563,9 @@ def pmc_summary(pmc_dir, out_md, command, title):
     """Summarises one `rocprofv3 --pmc <SQ/GRBM counters>` run of this benchmark
     per kernel: each counter summed over the kernel's dispatches, the MFMA work
     the hardware counted (SQ_INSTS_VALU_MFMA_MOPS_* x 512 flops), and the MFMA
-    busy fraction SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 256 CUs x 4
-    SIMDs), the formula rocprofiler-compute uses for MFMA utilisation."""
+    busy fraction SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / 8 x 1024 SIMDs).
+    GRBM_GUI_ACTIVE is summed over the 8 XCDs (it reads 8x the kernel's cycles
+    at 2.4 GHz), so GRBM_GUI_ACTIVE / 8 is the kernel's duration in cycles."""
     import csv
     import glob
     hits = glob.glob(os.path.join(pmc_dir, "**", "*counter_collection.csv"), recursive=True)
@@ -582,7 +583,7 @@ def pmc_summary(pmc_dir, out_md, command, title):
     lines = [f"# rocprofv3 counters: {title}", "", "Command (run from the repository root on one MI355X):", "",
              "```", command, "```", "",
              "Derived: `mfma_tflop` = (SQ_INSTS_VALU_MFMA_MOPS_F64 + _F32) x 512 flops summed over the "
-             "kernel's dispatches; `mfma_busy` = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 256 CUs x 4 SIMDs).",
+             "kernel's dispatches; `mfma_busy` = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / 8 XCDs x 256 CUs x 4 SIMDs).",
              "", "| kernel | dispatches | " + " | ".join(ctrs) + " | mfma_tflop | mfma_busy |",
              "|---|---:|" + "---:|" * (len(ctrs) + 2)]
     for name, v in rows[:16]:
@@ -590,7 +591,7 @@ def pmc_summary(pmc_dir, out_md, command, title):
         mops = v.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0) + v.get("SQ_INSTS_VALU_MFMA_MOPS_F32", 0.0)
         busy = v.get("SQ_VALU_MFMA_BUSY_CYCLES")
         gui = v.get("GRBM_GUI_ACTIVE")
-        util = f"{busy / (gui * 256 * 4):.3f}" if busy is not None and gui else "-"
+        util = f"{busy / (gui / 8 * 256 * 4):.3f}" if busy is not None and gui else "-"
         lines.append(f"| `{short}` | {len(disp[name])} | " + " | ".join(f"{v.get(c, 0.0):.4g}" for c in ctrs)
                      + f" | {mops * 512 / 1e12:.3f} | {util} |")
     with open(out_md, "w") as f:
