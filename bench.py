@@ -108,7 +108,8 @@ def batched_leg(dev, torch, args):
     sizes = np.minimum(sizes, 512)
     rhs = rng.integers(1, 9, cnt).astype(np.int32)
     once = [int(v) for v in args.batch_once.split(",")] if args.batch_once else None
-    plan = (dev.BatchPlan(sizes, rhs, prec="f64", small_limit=once[0], small_waves=once[1]) if once
+    plan = (dev.BatchPlan(sizes, rhs, prec="f64", small_limit=once[0], small_waves=once[1],
+                          medium_mode=once[2] if len(once) > 2 else 0) if once
             else dev.BatchPlan(sizes, rhs, prec="f64"))
     a = torch.zeros(plan.a_elems, dtype=torch.float64, device="cuda")
     b = torch.zeros(plan.b_elems, dtype=torch.float64, device="cuda")
@@ -718,7 +719,7 @@ def main():
     ap.add_argument("--lambdas", default="1,1e-2,1e-4,1e-6",
                     help="lambdas of the mixed-precision sweep (the first is also --lam's leg)")
     ap.add_argument("--no-f16", action="store_true")
-    ap.add_argument("--batch-once", default="", metavar="SMALL_LIMIT,SMALL_WAVES",
+    ap.add_argument("--batch-once", default="", metavar="SMALL_LIMIT,SMALL_WAVES[,MEDIUM_MODE]",
                     help="run one f64 graph launch of the batched workload with this configuration, print "
                          "its JSON and exit (for counter runs)")
     ap.add_argument("--batched-summary", nargs=4, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR", "ONCE_JSON"),

@@ -325,8 +325,9 @@ llh_status llh_dev_solve_mixed(llh_context* ctx, llh_precision prec, size_t n, c
  *
  * A plan fixes the sizes, buckets them (small systems: one workgroup each in
  * LDS, in the reference's exact operation order, from a persistent work queue;
- * larger ones: a blocked factor whose steps run as grouped launches over every
- * system still active), and captures the whole sequence in one hipGraph that
+ * larger ones: a blocked factor, fused per system in one persistent launch or
+ * run as grouped launches over every system still active, one per step), and
+ * captures the whole sequence in one hipGraph that
  * every llh_batch_solve replays.  llh_batch_autotune times the candidate
  * bucket bounds and launch shapes on real data and keeps the fastest.
  * ------------------------------------------------------------------------- */
@@ -336,6 +337,10 @@ typedef struct llh_batch_config {
   int32_t small_limit;   /* systems with n <= small_limit take the LDS path (<= 128 f64, <= 176 f32) */
   int32_t small_waves;   /* resident workgroups per CU of the persistent small-system kernels */
   int32_t ir_iterations; /* LLH_PREC_F32_REFINED: refinement steps (0 = 2) */
+  int32_t medium_mode;    /* the larger systems, when every one has n <= 512: 0 fused -- one workgroup per
+                             system through its whole factor and substitutions, from a work queue; 1 the
+                             factor fused so, then one substitution launch; 2 grouped launches, one per
+                             blocked step over every system (always so when some n > 512) */
 } llh_batch_config;
 
 /* prec: LLH_PREC_F64 (buffers of double), LLH_PREC_F32 (buffers of float), or

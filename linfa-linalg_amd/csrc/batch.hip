@@ -43,7 +43,7 @@ struct llh_batch_plan {
   std::vector<int32_t> n, m, lda;
   std::vector<uint64_t> a_off, b_off;
   size_t a_elems = 0, b_elems = 0;
-  llh_batch_config cfg{0, 0};
+  llh_batch_config cfg{0, 0, 0, 0};
   int cus = 256;
 
   // schedule (rebuilt when the configuration changes)
@@ -63,6 +63,9 @@ struct llh_batch_plan {
   int nclasses = 0;                      // small-system launches per pass
   int med_first = 0, nmed = 0;           // items of the blocked (medium) bucket
   bool lds_solve = false;                // every medium n <= kSolveLdsMaxN: one substitution launch
+  bool med_fused = false;                // the medium bucket's factor runs as one fused per-system launch
+  bool med_fused_solve = false;          // ... and its substitutions in the same launch
+  int med_counter = 0, med_grid = 0;     // its work-queue counter and resident grid
   int ncounters = 0;
   size_t linv_elems = 0;
 
@@ -138,6 +141,16 @@ hipError_t record_pass(llh_batch_plan* p, hipStream_t s, int pass) {
     if ((e = launch_small_batch<T>(base, items + sl.first, sl.count, sl.nmax, sl.grid,
                                    sl.counter + pass * p->nclasses, p->side, pass == 0 ? 1 : 2)) != hipSuccess)
       return e;
+  if (pass == 0 && p->med_fused) {
+    // every medium system's factor (and substitutions), one workgroup each
+    if ((e = launch_med<T>(base, items + p->med_first, p->nmed, p->med_counter, p->med_grid,
+                           p->med_fused_solve, s)) != hipSuccess)
+      return e;
+    if (!p->med_fused_solve)
+      if ((e = launch_grp_solve<T>(base, items + p->med_first, p->nmed, s)) != hipSuccess) return e;
+    if ((e = hipEventRecord(p->join, p->side)) != hipSuccess) return e;
+    return hipStreamWaitEvent(s, p->join, 0);
+  }
   for (const auto& st : p->steps) {
     if (pass == 0) {
       if ((e = launch_grp_diag_fast<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
@@ -199,6 +212,7 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
   const int smax = p->prec == LLH_PREC_F64 ? kSmallMaxD : kSmallMaxS;
   p->cfg.small_limit = std::max(0, std::min(p->cfg.small_limit, smax));
   p->cfg.small_waves = std::max(1, std::min(p->cfg.small_waves, 16));
+  p->cfg.medium_mode = std::max(0, std::min(p->cfg.medium_mode, 2));
   std::vector<int> order(p->count);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return p->n[x] > p->n[y]; });
@@ -246,6 +260,14 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
   p->med_first = med_first;
   p->nmed = (int)items.size() - med_first;
   p->lds_solve = maxn <= kSolveLdsMaxN;
+  p->med_fused = p->lds_solve && p->nmed > 0 && p->cfg.medium_mode != 2;
+  p->med_fused_solve = p->med_fused && p->cfg.medium_mode == 0;
+  p->med_counter = p->ncounters;
+  if (p->med_fused) {
+    p->ncounters += 1;
+    p->med_grid = p->cus * (p->prec == LLH_PREC_F64 ? med_resident_per_cu<double>(p->med_fused_solve)
+                                                    : med_resident_per_cu<float>(p->med_fused_solve));
+  }
   std::vector<StepWork> sw;
   std::vector<TileWork> tw;
   p->steps.clear();
@@ -376,7 +398,7 @@ llh_status llh_batch_plan_create(llh_context* ctx, llh_precision prec, size_t co
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) p->cus = prop.multiProcessorCount;
-  p->cfg = config ? *config : llh_batch_config{prec == LLH_PREC_F64 ? kSmallMaxD : kSmallMaxS, 4, 0};
+  p->cfg = config ? *config : llh_batch_config{prec == LLH_PREC_F64 ? kSmallMaxD : kSmallMaxS, 4, 0, 0};
   // packed layout: 256-byte aligned starts, lda rounded up to 4
   const size_t es = esize(p), align = 256 / es;
   size_t ao = 0, bo = 0;
@@ -430,7 +452,7 @@ llh_status llh_batch_layout(const llh_batch_plan* p, uint64_t* a_off, int32_t* l
 }
 
 llh_batch_config llh_batch_get_config(const llh_batch_plan* p) {
-  return bad_plan(p) ? llh_batch_config{0, 0, 0} : p->cfg;
+  return bad_plan(p) ? llh_batch_config{0, 0, 0, 0} : p->cfg;
 }
 
 llh_status llh_batch_solve(llh_context* ctx, llh_batch_plan* p, void* a, void* b, int32_t* status,
@@ -465,40 +487,46 @@ llh_status llh_batch_autotune(llh_context* ctx, llh_batch_plan* p, const void* a
   llh_status st = LLH_OK;
   const int limits[] = {32, 64, smax};
   const int waves[] = {1, 2, 4, 8};
-  for (int lim : limits) {
+  // candidates: bucket bound x resident small workgroups x the medium bucket's
+  // form (fused with its substitutions, fused factor, grouped steps)
+  std::vector<llh_batch_config> cands;
+  for (int lim : limits)
     for (int wv : waves) {
       if (lim > 64 && wv > 1) continue;  // one resident 128-class workgroup per CU fills its LDS
-      p->cfg = {lim, wv, keep.ir_iterations};
-      st = build_schedule(ctx, p, err);
-      if (st != LLH_OK) break;
-      float ms_best = 1e30f;
-      for (int rep = 0; rep < 2 && st == LLH_OK; ++rep) {
-        e = hipMemcpyAsync(wa, a, ab, hipMemcpyDeviceToDevice, ctx->stream);
-        if (e == hipSuccess && bb) e = hipMemcpyAsync(wb, b, bb, hipMemcpyDeviceToDevice, ctx->stream);
-        if (e == hipSuccess) e = hipEventRecord(t0, ctx->stream);
-        if (e != hipSuccess) {
-          st = hip_fail(err, e, "autotune");
-          break;
-        }
-        st = run(ctx, p, wa, wb, status, false, err);
-        if (st != LLH_OK) break;
-        e = hipEventRecord(t1, ctx->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(t1);
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
-        if (e != hipSuccess) {
-          st = hip_fail(err, e, "autotune timing");
-          break;
-        }
-        ms_best = std::min(ms_best, ms);
+      for (int mode = 0; mode < 3; ++mode) cands.push_back({lim, wv, keep.ir_iterations, mode});
+    }
+  for (const llh_batch_config& cand : cands) {
+    p->cfg = cand;
+    st = build_schedule(ctx, p, err);
+    if (st != LLH_OK) break;
+    // without a fusable medium bucket the three forms are the same schedule
+    if (cand.medium_mode != 2 && !(p->lds_solve && p->nmed > 0)) continue;
+    float ms_best = 1e30f;
+    for (int rep = 0; rep < 2 && st == LLH_OK; ++rep) {
+      e = hipMemcpyAsync(wa, a, ab, hipMemcpyDeviceToDevice, ctx->stream);
+      if (e == hipSuccess && bb) e = hipMemcpyAsync(wb, b, bb, hipMemcpyDeviceToDevice, ctx->stream);
+      if (e == hipSuccess) e = hipEventRecord(t0, ctx->stream);
+      if (e != hipSuccess) {
+        st = hip_fail(err, e, "autotune");
+        break;
       }
+      st = run(ctx, p, wa, wb, status, false, err);
       if (st != LLH_OK) break;
-      if (ms_best < best_t) {
-        best_t = ms_best;
-        best = p->cfg;
+      e = hipEventRecord(t1, ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(t1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+      if (e != hipSuccess) {
+        st = hip_fail(err, e, "autotune timing");
+        break;
       }
+      ms_best = std::min(ms_best, ms);
     }
     if (st != LLH_OK) break;
+    if (ms_best < best_t) {
+      best_t = ms_best;
+      best = p->cfg;
+    }
   }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);

@@ -7,6 +7,7 @@
 #include <algorithm>
 
 #include "llh_common.h"
+#include "k_diag_impl.h"
 #include "k_gemm_impl.h"
 #include "k_trsm_impl.h"
 #include "llh_kernels.h"
@@ -17,6 +18,17 @@ namespace {
 template <typename T>
 __device__ __forceinline__ bool failed(const BatchBase<T>& bb, const BatchItem& it) {
   return bb.status[it.id] != kNoFail;
+}
+
+// Work item of workgroup b in a launch of `count` workgroups, chosen so that
+// each XCD takes a contiguous run of the work list: workgroups are dealt to the
+// 8 XCDs round-robin (b to XCD b % 8), and XCD x gets items
+// [x q + min(x, r), ...) with q = count / 8, r = count % 8 -- a bijection on
+// [0, count).  The tiles of one system sit together in the list, so they run
+// on one XCD and read their panel blocks from its L2, not from HBM.
+__device__ __forceinline__ int xcd_item(int b, int count) {
+  const int x = b & 7, q = count >> 3, r = count & 7;
+  return x * q + min(x, r) + (b >> 3);
 }
 
 // inv(L_kk) of the block at (k0, k0) into the matrix's inverse slot k0 / 64.
@@ -44,7 +56,7 @@ __global__ __launch_bounds__(256) void grp_panel_kernel(const BatchBase<T>* __re
                                                         const BatchItem* __restrict__ items,
                                                         const TileWork* __restrict__ work, int count) {
   if ((int)blockIdx.x >= count) return;
-  const TileWork w = work[blockIdx.x];
+  const TileWork w = work[xcd_item(blockIdx.x, count)];
   const BatchItem it = items[w.item];
   const BatchBase<T> bb = *base;
   if (failed(bb, it)) return;
@@ -60,7 +72,7 @@ __global__ __launch_bounds__(256) void grp_syrk_kernel(const BatchBase<T>* __res
                                                        const BatchItem* __restrict__ items,
                                                        const TileWork* __restrict__ work, int count) {
   if ((int)blockIdx.x >= count) return;
-  const TileWork w = work[blockIdx.x];
+  const TileWork w = work[xcd_item(blockIdx.x, count)];
   const BatchItem it = items[w.item];
   const BatchBase<T> bb = *base;
   if (failed(bb, it)) return;
@@ -190,16 +202,13 @@ __global__ __launch_bounds__(256) void grp_bwd_kernel(const BatchBase<T>* __rest
 //                      64-column chunks i left of k (L's block row k, read
 //                      along its rows, the transpose taken in LDS).
 // Thread (r, c) owns row (or column) r of the 64-block and columns c, c + 4.
+// solve_system runs it for system `it` with the caller's LDS: Bs holds
+// kSolveLdsMaxN x kSolveCols values, Ts a 64 x 65 tile.
+constexpr int kSolveCols = 8;
 template <typename T>
-__global__ __launch_bounds__(256) void grp_solve_kernel(const BatchBase<T>* __restrict__ base,
-                                                        const BatchItem* __restrict__ items, int count) {
-  constexpr int MC = 8, S = kNB + 1;
-  if ((int)blockIdx.x >= count) return;
-  const BatchItem it = items[blockIdx.x];
-  const BatchBase<T> bb = *base;
-  if (failed(bb, it)) return;
-  __shared__ T Bs[kSolveLdsMaxN * MC];
-  __shared__ T Ts[kNB * S];
+__device__ __forceinline__ void solve_system(const BatchBase<T>& bb, const BatchItem& it, T* __restrict__ Bs,
+                                             T* __restrict__ Ts) {
+  constexpr int MC = kSolveCols, S = kNB + 1;
   const int tid = threadIdx.x, n = it.n, m = it.m, nb = (n + kNB - 1) / kNB;
   const int64_t lda = it.lda;
   const T* a = bb.a + it.a_off;
@@ -355,6 +364,86 @@ __global__ __launch_bounds__(256) void grp_solve_kernel(const BatchBase<T>* __re
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void grp_solve_kernel(const BatchBase<T>* __restrict__ base,
+                                                        const BatchItem* __restrict__ items, int count) {
+  if ((int)blockIdx.x >= count) return;
+  const BatchItem it = items[blockIdx.x];
+  const BatchBase<T> bb = *base;
+  if (failed(bb, it)) return;
+  __shared__ T Bs[kSolveLdsMaxN * kSolveCols];
+  __shared__ T Ts[kNB * (kNB + 1)];
+  solve_system<T>(bb, it, Bs, Ts);
+}
+
+// The phases of the fused per-system kernel run one after another, so they
+// share one LDS buffer.
+template <typename T>
+union MedLds {
+  DiagLds<T> diag;
+  struct {
+    NtSlab<T, 64> as[2];
+    NtSlab<T, 64> bs[2];
+  } tile;
+  struct {
+    T bs[kSolveLdsMaxN * kSolveCols];
+    T ts[kNB * (kNB + 1)];
+  } solve;
+};
+
+// The medium bucket fused per system (every n <= kSolveLdsMaxN): one workgroup
+// takes systems from a work queue, largest first (the items are sorted by
+// decreasing n), and runs each through its whole blocked factor -- per 64-column
+// step the diagonal block with its inverse, the panel as one MFMA product with
+// that inverse, the trailing update on MFMA over 64 x 64 lower tiles -- and,
+// with SOLVE, its forward and back substitution.  It is the device code of the
+// grouped launches, in the same order for each system, so the results are bit
+// for bit those of the grouped schedule; the difference is the traversal: the
+// system stays in the workgroup's L2 between its steps instead of every step
+// streaming the whole batch through HBM.  A failed diagonal block ends the
+// system (status holds the row, as in the grouped path).
+template <typename T, bool SOLVE>
+__global__ __launch_bounds__(256, 2) void med_kernel(const BatchBase<T>* __restrict__ base,
+                                                  const BatchItem* __restrict__ items, int count, int counter) {
+  __shared__ __attribute__((aligned(16))) MedLds<T> sm;
+  __shared__ int s_next;
+  const BatchBase<T> bb = *base;
+  NtSlab<T, 64>* const as = sm.tile.as;
+  NtSlab<T, 64>* const bs = sm.tile.bs;
+  for (;;) {
+    __syncthreads();  // the previous system's LDS use and the read of s_next are done
+    if (threadIdx.x == 0) s_next = atomicAdd(bb.counters + counter, 1);
+    __syncthreads();
+    const int q = s_next;
+    if (q >= count) return;
+    const BatchItem it = items[q];
+    const int n = it.n;
+    const int64_t lda = it.lda;
+    T* a = bb.a + it.a_off;
+    bool ok = true;
+    for (int k0 = 0; k0 < n; k0 += kNB) {
+      const int b = min(kNB, n - k0);
+      T* li = bb.linv + it.linv_off + (int64_t)(k0 / kNB) * kNB * kNB;
+      if (diag_quad_block<T, T>(a + (int64_t)k0 * lda + k0, lda, b, li, bb.status + it.id, k0, sm.diag)) {
+        ok = false;  // uniform: every thread read the same failing column
+        break;
+      }
+      __syncthreads();  // L_kk and its inverse written; the LDS is free for the tiles
+      const int below = n - k0 - b;
+      T* p = a + (int64_t)(k0 + b) * lda + k0;
+      for (int m0 = 0; m0 < below; m0 += 64)
+        nt_tile_in<T, 64, 64, 2, 2, false, true>(below, b, b, T(1), p, lda, li, kNB, T(0), p, lda, m0, 0, as, bs);
+      __syncthreads();  // the whole panel is written before the update reads it
+      for (int m0 = 0; m0 < below; m0 += 64)
+        for (int n0 = 0; n0 <= m0; n0 += 64)
+          nt_tile_in<T, 64, 64, 2, 2, true, true>(below, below, b, T(-1), p, lda, p, lda, T(1), p + b, lda, m0,
+                                                  n0, as, bs);
+      __syncthreads();  // the next diagonal block is updated
+    }
+    if (SOLVE && ok) solve_system<T>(bb, it, sm.solve.bs, sm.solve.ts);
+  }
+}
+
 // Elementwise steps of the mixed-precision batch (pointers read from the device
 // base, so a replayed graph follows the caller's buffers).
 __global__ void mixed_flat_kernel(const MixedBase* __restrict__ base, int op, int64_t elems) {
@@ -440,6 +529,31 @@ hipError_t launch_grp_solve(const BatchBase<T>* base, const BatchItem* items, in
 }
 template hipError_t launch_grp_solve<double>(const BatchBase<double>*, const BatchItem*, int, hipStream_t);
 template hipError_t launch_grp_solve<float>(const BatchBase<float>*, const BatchItem*, int, hipStream_t);
+
+template <typename T>
+int med_resident_per_cu(bool solve) {
+  int nb = 0;
+  const hipError_t e = solve ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, med_kernel<T, true>, 256, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, med_kernel<T, false>, 256, 0);
+  return e == hipSuccess && nb > 0 ? nb : 1;
+}
+template int med_resident_per_cu<double>(bool);
+template int med_resident_per_cu<float>(bool);
+
+template <typename T>
+hipError_t launch_med(const BatchBase<T>* base, const BatchItem* items, int count, int counter, int grid,
+                      bool solve, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  grid = std::max(1, std::min(grid, count));
+  if (solve)
+    hipLaunchKernelGGL((med_kernel<T, true>), dim3(grid), dim3(256), 0, s, base, items, count, counter);
+  else
+    hipLaunchKernelGGL((med_kernel<T, false>), dim3(grid), dim3(256), 0, s, base, items, count, counter);
+  return hipGetLastError();
+}
+template hipError_t launch_med<double>(const BatchBase<double>*, const BatchItem*, int, int, int, bool,
+                                       hipStream_t);
+template hipError_t launch_med<float>(const BatchBase<float>*, const BatchItem*, int, int, int, bool, hipStream_t);
 
 hipError_t launch_mixed_flat(const MixedBase* mb, int op, int64_t elems, hipStream_t s) {
   if (elems <= 0) return hipSuccess;

@@ -80,10 +80,18 @@ constexpr int mfma_lane_k() { return sizeof(TI) == 2 ? 8 : 1; }
 //
 // TI is the operand type: T itself, or _Float16 with T = float (f16 products
 // accumulated in f32; one MFMA then takes a lane's whole 8-element fragment).
+//
+// nt_tile_in takes the two double-buffered LDS slabs from the caller (the
+// types NtSlab<TI, BM> for A and NtSlab<TI, BN> for B, two buffers each), so a
+// kernel that runs other phases between tiles can share the memory; nt_tile
+// declares them itself.
+template <typename TI, int ROWS>
+using NtSlab = TI[ROWS * 9 * (16 / sizeof(TI))];
+
 template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool VEC, bool PRE = false, typename TI = T>
-__device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const TI* A, int64_t lda,
-                                        const TI* __restrict__ B, int64_t ldb, T beta, T* C, int64_t ldc,
-                                        int m0, int n0) {
+__device__ __forceinline__ void nt_tile_in(int M, int N, int K, T alpha, const TI* A, int64_t lda,
+                                           const TI* __restrict__ B, int64_t ldb, T beta, T* C, int64_t ldc,
+                                           int m0, int n0, NtSlab<TI, BM>* As, NtSlab<TI, BN>* Bs) {
   constexpr int NTHR = WM * WN * 64;
   constexpr int TM = BM / WM, TN = BN / WN;
   constexpr int FM = TM / 16, FN = TN / 16;
@@ -102,8 +110,8 @@ __device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const TI* 
   using acc_t = typename MF::acc_t;
   typedef TI vec_t __attribute__((ext_vector_type(KPL)));
 
-  __shared__ __attribute__((aligned(16))) TI As[2][BM * LDK];
-  __shared__ __attribute__((aligned(16))) TI Bs[2][BN * LDK];
+  static_assert(sizeof(NtSlab<TI, BM>) == BM * LDK * sizeof(TI) && sizeof(NtSlab<TI, BN>) == BN * LDK * sizeof(TI),
+                "LDS slab types match the tile");
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -244,6 +252,15 @@ __device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const TI* 
           *c = (beta == T(0) || (PRE && pre)) ? v : beta * *c + v;
         }
       }
+}
+
+template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool VEC, bool PRE = false, typename TI = T>
+__device__ __forceinline__ void nt_tile(int M, int N, int K, T alpha, const TI* A, int64_t lda,
+                                        const TI* __restrict__ B, int64_t ldb, T beta, T* C, int64_t ldc,
+                                        int m0, int n0) {
+  __shared__ __attribute__((aligned(16))) NtSlab<TI, BM> As[2];
+  __shared__ __attribute__((aligned(16))) NtSlab<TI, BN> Bs[2];
+  nt_tile_in<T, BM, BN, WM, WN, TRI, VEC, PRE, TI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs);
 }
 
 }  // namespace llh

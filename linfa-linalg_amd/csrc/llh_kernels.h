@@ -114,6 +114,15 @@ hipError_t launch_fill_status(int32_t* status, int64_t count, hipStream_t s);
 constexpr int kSolveLdsMaxN = 512;
 template <typename T>
 hipError_t launch_grp_solve(const BatchBase<T>* base, const BatchItem* items, int count, hipStream_t s);
+// The medium bucket fused per system (k_batch.hip): factor (and with solve, the
+// substitutions) of `count` items, one workgroup per system from the work
+// queue bb.counters[counter] (zero before the launch), every n <= kSolveLdsMaxN.
+template <typename T>
+hipError_t launch_med(const BatchBase<T>* base, const BatchItem* items, int count, int counter, int grid,
+                      bool solve, hipStream_t s);
+// Workgroups of that kernel resident per CU (from the occupancy calculator).
+template <typename T>
+int med_resident_per_cu(bool solve);
 
 // k_diag.hip: fast factor of one b x b (b <= 64) diagonal block in place, and
 // its inverse into inv (64 x 64 row-major); a failure at block row f sets

@@ -44,7 +44,7 @@ class RefineInfo(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("small_limit", ctypes.c_int32), ("small_waves", ctypes.c_int32),
-                ("ir_iterations", ctypes.c_int32)]
+                ("ir_iterations", ctypes.c_int32), ("medium_mode", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):

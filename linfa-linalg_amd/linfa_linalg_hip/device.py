@@ -269,17 +269,18 @@ class BatchPlan:
     buffers, each A_i factored in f32 from a copy (A_i left unchanged) and B_i
     refined ir_iterations times in f64, still one launch."""
 
-    def __init__(self, n, m, prec="f64", small_limit=None, small_waves=None, ir_iterations=None):
+    def __init__(self, n, m, prec="f64", small_limit=None, small_waves=None, ir_iterations=None,
+                 medium_mode=None):
         import numpy as np
         self._c = context()
         self.n = np.ascontiguousarray(n, dtype=np.int32)
         self.m = np.ascontiguousarray(m, dtype=np.int32)
         self.prec = prec
         cfg = None
-        if small_limit is not None or small_waves is not None or ir_iterations is not None:
+        if any(v is not None for v in (small_limit, small_waves, ir_iterations, medium_mode)):
             cfg = _abi.BatchConfig(int(small_limit if small_limit is not None else (128 if prec == "f64" else 176)),
                                    int(small_waves if small_waves is not None else 4),
-                                   int(ir_iterations or 0))
+                                   int(ir_iterations or 0), int(medium_mode or 0))
         self._ptr = ctypes.c_void_p()
         err = _abi.LlhError()
         i32p = ctypes.POINTER(ctypes.c_int32)
@@ -309,7 +310,8 @@ class BatchPlan:
     @property
     def config(self):
         c = self._c._lib.llh_batch_get_config(self._ptr)
-        return {"small_limit": c.small_limit, "small_waves": c.small_waves, "ir_iterations": c.ir_iterations}
+        return {"small_limit": c.small_limit, "small_waves": c.small_waves, "ir_iterations": c.ir_iterations,
+                "medium_mode": c.medium_mode}
 
     def solve(self, a, b, status):
         err = _abi.LlhError()

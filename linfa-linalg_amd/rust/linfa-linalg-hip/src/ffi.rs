@@ -96,6 +96,7 @@ pub struct llh_batch_config {
     pub small_limit: i32,
     pub small_waves: i32,
     pub ir_iterations: i32,
+    pub medium_mode: i32,
 }
 
 #[repr(C)]

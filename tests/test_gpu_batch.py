@@ -191,3 +191,50 @@ def test_batch_f32_refined_failure_is_isolated():
     for i in (0, 3, 4):
         x_ref = orc.solvec(np.array(mats[i]), bs[i])
         assert np.abs(xs[i] - x_ref).max() <= 1e-12 * max(1, np.abs(x_ref).max())
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f32ir"])
+def test_batch_medium_forms_bitwise_equal(prec):
+    """The medium bucket's three forms (medium_mode 0: one workgroup per system
+    through its factor and substitutions; 1: the factor so, then one
+    substitution launch; 2: grouped launches, one per blocked step) run the same
+    device code in the same order per system, so factors, solutions and status
+    agree bit for bit -- with a failing system among them, and on a replay."""
+    import torch
+    from linfa_linalg_hip import device as dev
+    rng = np.random.default_rng(21)
+    sizes = [33, 64, 65, 127, 128, 129, 200, 256, 257, 300, 383, 448, 511, 512, 7, 20] + \
+        [int(x) for x in rng.integers(33, 513, 40)]
+    rhs = [0, 1, 8, 9, 16, 17, 3, 2, 1, 5, 8, 4, 20, 1, 2, 3] + [int(x) for x in rng.integers(0, 9, 40)]
+    dtype = np.float32 if prec == "f32" else np.float64
+    mats, bs = make_batch(sizes, rhs, dtype, 23)
+    mats[9][211, 211] = -1.0  # n = 300 fails at row 211 (third block)
+    outs = []
+    for mode in (0, 1, 2):
+        plan = dev.BatchPlan(sizes, rhs, prec=prec, small_limit=32, medium_mode=mode)
+        assert plan.config["medium_mode"] == mode
+        a, b = pack(plan, mats, bs, dtype)
+        ta = torch.from_numpy(a).cuda()
+        res = []
+        for _ in range(2):
+            ta.copy_(torch.from_numpy(a))
+            tb = torch.from_numpy(b).cuda()
+            st = torch.zeros(len(sizes), dtype=torch.int32, device="cuda")
+            plan.solve(ta, tb, st)
+            res.append((ta.cpu().numpy(), tb.cpu().numpy(), st.cpu().numpy()))
+        assert all(np.array_equal(x, y) for x, y in zip(res[0], res[1])), mode  # replay
+        outs.append(res[0])
+        plan.close()
+    expect = [-1] * len(sizes)
+    expect[9] = 211
+    for mode, (a, b, st) in enumerate(outs):
+        assert list(st) == expect, mode
+        assert np.array_equal(a, outs[2][0]) and np.array_equal(b, outs[2][1]), mode
+    # and the solutions are the oracle's to the blocked path's accuracy
+    _, xs = unpack(plan, outs[0][0], outs[0][1], mats, bs)
+    eps = np.finfo(dtype).eps if prec != "f32ir" else 1e-15
+    for i, (am, bm, x) in enumerate(zip(mats, bs, xs)):
+        if i == 9 or bm.size == 0:
+            continue
+        x_ref = orc.solvec(np.array(am, dtype=np.float64), np.array(bm, dtype=np.float64))
+        assert np.abs(x - x_ref).max() <= 1e3 * eps * max(1.0, np.abs(x_ref).max()), (i, sizes[i])
