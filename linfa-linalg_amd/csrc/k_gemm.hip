@@ -242,16 +242,19 @@ __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_ker
   const int m0 = tm * BM, n0 = tn * BN;
   if (TRI && m0 + BM - 1 < n0) return;
 
-  // C prefetch for the trailing SYRK: f64 on 256 x 128 tiles (the 128 x 128
-  // f64 tile has no registers left) and f16 operands on 128 x 128 tiles.
-  // Measured at n = 32256, k = 512: f64 53 -> 57, f16 199 -> 348 TFLOP/s.  Not
+  // C prefetch for the triangular updates (the trailing SYRK and the look-ahead
+  // panel's columns): f64 on 256 x 128 tiles (the 128 x 128 f64 tile has no
+  // registers left) and f16 operands on 128 x 128 tiles.  Measured on the SYRK
+  // at n = 32256, k = 512: f64 53 -> 57, f16 199 -> 348 TFLOP/s; on the
+  // look-ahead update as well, n = 32768: f64 solve 219.7 -> 217.4 ms, FP16+IR
+  // 81.0 -> 79.6 ms.  Not
   // for f32 operands (88 -> 107 TFLOP/s): folding C into the f32 sum mid-way
   // costs the accuracy that refinement then pays for (11 instead of 3 steps at
   // cond ~4e6), and holding C to the end does not fit the registers.  Seeding
   // the f32 accumulators with C before the first slab is no better: 119 instead
   // of 104 TFLOP/s alone, no gain inside the factor (149.6 -> 148.4 ms at
   // n = 32768), and 29 instead of 4 refinement steps at lambda = 1e-6.
-  constexpr bool PRE = SQ && ((BM == 256 && sizeof(T) == 8) || sizeof(TI) == 2);
+  constexpr bool PRE = TRI && ((BM == 256 && sizeof(T) == 8) || sizeof(TI) == 2);
   nt_tile<T, BM, BN, WM, WN, TRI, VEC, PRE, TI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
 }
 
