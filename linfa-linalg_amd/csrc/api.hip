@@ -363,12 +363,9 @@ llh_status solve_on_device(llh_context* ctx, size_t n, T* da, size_t m, T* db, l
     CHECK_HIP(potrf_enqueue<T>(da, (int64_t)n, (int)n, ctx->d_info, false, dinv, ctx->stream, ctx->profiler(),
                                                false, ctx_lookahead(ctx)),
               "potrf");
-    CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, dinv, false, nullptr,
-                              &sync, ctx->stream, ctx->profiler(), ctx->d_info),
-              "forward solve");
-    CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, dinv, true, nullptr,
-                              &sync, ctx->stream, ctx->profiler(), ctx->d_info),
-              "back solve");
+    CHECK_HIP(solve_pair_enqueue<T>((int)n, da, (int64_t)n, (int)m, db, (int64_t)m, dinv, &sync, ctx->stream,
+                                    ctx->profiler(), ctx->d_info),
+              "solve");
   }
   CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   return LLH_OK;
@@ -530,12 +527,9 @@ llh_status dev_solvec(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T*
     // on the factor's status word: after NotPositiveDefinite every solve kernel
     // is a no-op and B keeps the caller's values, as the reference returns
     // before touching b (cholesky.rs:140).
-    CHECK_HIP(trsm_enqueue<T>(false, false, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, false,
-                              nullptr, &sync, ctx->stream, ctx->profiler(), ctx->d_info),
-              "forward solve");
-    CHECK_HIP(trsm_enqueue<T>(true, true, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, true, nullptr,
-                              &sync, ctx->stream, ctx->profiler(), ctx->d_info),
-              "back solve");
+    CHECK_HIP(solve_pair_enqueue<T>((int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, &sync, ctx->stream,
+                                    ctx->profiler(), ctx->d_info),
+              "solve");
   }
   CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");

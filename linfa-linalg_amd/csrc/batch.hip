@@ -76,12 +76,11 @@ struct llh_batch_plan {
   void* d_base = nullptr;
   void* d_mbase = nullptr;  // MixedBase (refined mode)
   void* d_linv = nullptr;
-  // refined mode: f32 copy of A, f32 right-hand side / correction, f64 copies
-  // of B and of the residual
+  // refined mode: f32 copy of A, f32 right-hand side / residual / correction,
+  // f64 copy of B
   float* a32 = nullptr;
   float* x32 = nullptr;
   double* bk64 = nullptr;
-  double* r64 = nullptr;
   int32_t* d_counters = nullptr;
   size_t linv_bytes = 0;
 
@@ -186,19 +185,19 @@ hipError_t record(llh_batch_plan* p, hipStream_t s) {
     if ((e = record_pass<T>(p, s, 0)) != hipSuccess) return e;
   } else {
     // f32 factor and solve from copies, then refinement in f64:
-    // x = x32; repeat { r = b - A x (f64); x32 = solve32(r); x += x32 }
+    // x = x32; repeat { x32 = r = b - A x (f64 sums); x32 = solve32(x32); x += x32 },
+    // the updates of x skipping systems whose factor failed (their B stays the caller's)
     const auto* mb = static_cast<const MixedBase*>(p->d_mbase);
     const auto* items = static_cast<const BatchItem*>(p->d_items);
     const auto* sw = static_cast<const StepWork*>(p->d_step);
     if ((e = launch_mixed_flat(mb, 0, (int64_t)p->a_elems, s)) != hipSuccess) return e;
     if ((e = launch_mixed_flat(mb, 1, (int64_t)p->b_elems, s)) != hipSuccess) return e;
     if ((e = record_pass<T>(p, s, 0)) != hipSuccess) return e;
-    if ((e = launch_mixed_flat(mb, 2, (int64_t)p->b_elems, s)) != hipSuccess) return e;
+    if ((e = launch_grp_update(mb, items, sw + p->res_off, p->nres, false, s)) != hipSuccess) return e;
     for (int it = 1; it <= ir_steps(p); ++it) {
       if ((e = launch_grp_residual(mb, items, sw + p->res_off, p->nres, s)) != hipSuccess) return e;
-      if ((e = launch_mixed_flat(mb, 3, (int64_t)p->b_elems, s)) != hipSuccess) return e;
       if ((e = record_pass<T>(p, s, it)) != hipSuccess) return e;
-      if ((e = launch_mixed_flat(mb, 4, (int64_t)p->b_elems, s)) != hipSuccess) return e;
+      if ((e = launch_grp_update(mb, items, sw + p->res_off, p->nres, true, s)) != hipSuccess) return e;
     }
   }
   hipLaunchKernelGGL((batch_finish_from_base<T>), dim3((cnt + 255) / 256), dim3(256), 0, s, base, cnt);
@@ -350,7 +349,7 @@ llh_status run(llh_context* ctx, llh_batch_plan* p, void* a, void* b, int32_t* s
   if (refined(p)) {
     // host staging of the two bases lives in the plan: the copies are async
     p->h_base32 = BatchBase<float>{p->a32, p->x32, static_cast<float*>(p->d_linv), status, p->d_counters};
-    p->h_mbase = MixedBase{static_cast<const double*>(a), static_cast<double*>(b), p->a32, p->x32, p->bk64, p->r64,
+    p->h_mbase = MixedBase{static_cast<const double*>(a), static_cast<double*>(b), p->a32, p->x32, p->bk64,
                            status};
     CHECK_HIP(hipMemcpyAsync(p->d_base, &p->h_base32, sizeof(p->h_base32), hipMemcpyHostToDevice, ctx->stream),
               "batch base");
@@ -424,7 +423,6 @@ llh_status llh_batch_plan_create(llh_context* ctx, llh_precision prec, size_t co
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->a32), std::max<size_t>(ao, 16) * 4);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->x32), std::max<size_t>(bo, 16) * 4);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->bk64), std::max<size_t>(bo, 16) * 8);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->r64), std::max<size_t>(bo, 16) * 8);
   }
   if (e != hipSuccess) {
     llh_batch_plan_destroy(p);
@@ -546,7 +544,7 @@ void llh_batch_plan_destroy(llh_batch_plan* p) {
   if (p->d_linv) (void)hipFree(p->d_linv);
   if (p->d_base) (void)hipFree(p->d_base);
   if (p->d_mbase) (void)hipFree(p->d_mbase);
-  for (void* q : {(void*)p->a32, (void*)p->x32, (void*)p->bk64, (void*)p->r64})
+  for (void* q : {(void*)p->a32, (void*)p->x32, (void*)p->bk64})
     if (q) (void)hipFree(q);
   if (p->fork) (void)hipEventDestroy(p->fork);
   if (p->join) (void)hipEventDestroy(p->join);

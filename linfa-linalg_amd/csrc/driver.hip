@@ -1,6 +1,7 @@
 // Host drivers: the blocked right-looking POTRF and the blocked triangular
 // solves, as sequences of kernel launches on one stream.
 #include <algorithm>
+#include <climits>
 
 #include "llh_driver.h"
 #include "llh_kernels.h"
@@ -240,6 +241,29 @@ hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t ld
   return hipSuccess;
 }
 
+// The forward and back substitution with the factor L (lower triangle of a)
+// and the inverses of its diagonal blocks (dinv, as the factor leaves them).
+// Up to kSolvePairLdsMaxN rows and kSolveCols right-hand sides it is one
+// single-workgroup launch with B in LDS (launch_solve_lds), otherwise the two
+// one-launch triangles (trsm_enqueue).  Measured for one right-hand side, both
+// triangles: n = 129 / 192 / 256 / 384 / 512 take 165 / 186 / 255 / 420 / 607 us
+// per solvec in one workgroup against 199 / 204 / 253 / 364 / 466 us with the
+// triangles, whose flag hops (~7 us per 64 rows) cost less than one workgroup's
+// pass over L beyond 256 rows.
+constexpr int kSolvePairLdsMaxN = 256;
+template <typename T>
+hipError_t solve_pair_enqueue(int n, const T* a, int64_t lda, int m, T* b, int64_t ldb, const T* dinv,
+                              SolveSync* sync, hipStream_t s, Profiler* prof, const int32_t* gate) {
+  const double es = (double)sizeof(T);
+  if (n <= kSolvePairLdsMaxN && m <= kSolveCols && dinv != nullptr && lda <= INT32_MAX) {
+    LLH_PROF(kSolveGemm, 2.0 * n * n * m, es * ((double)n * n + 2.0 * n * m),
+             (launch_solve_lds<T>(n, m, a, lda, dinv, b, ldb, gate, s)));
+    return hipSuccess;
+  }
+  LLH_TRY(trsm_enqueue<T>(false, false, n, a, lda, m, b, ldb, dinv, false, nullptr, sync, s, prof, gate));
+  return trsm_enqueue<T>(true, true, n, a, lda, m, b, ldb, dinv, true, nullptr, sync, s, prof, gate);
+}
+
 namespace {
 
 // Half of m rounded to whole kNB blocks (at least one block).
@@ -313,6 +337,10 @@ template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool,
                                           Profiler*, bool, const Lookahead*, _Float16*);
 template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, float*, hipStream_t, Profiler*,
                                          bool, const Lookahead*, _Float16*);
+template hipError_t solve_pair_enqueue<double>(int, const double*, int64_t, int, double*, int64_t, const double*,
+                                               SolveSync*, hipStream_t, Profiler*, const int32_t*);
+template hipError_t solve_pair_enqueue<float>(int, const float*, int64_t, int, float*, int64_t, const float*,
+                                              SolveSync*, hipStream_t, Profiler*, const int32_t*);
 template hipError_t trsm_enqueue<double>(bool, bool, int, const double*, int64_t, int, double*, int64_t,
                                          const double*, bool, double*, SolveSync*, hipStream_t, Profiler*,
                                          const int32_t*);

@@ -5,6 +5,7 @@
 // sequence of launches (captured in one hipGraph).  A matrix whose status word
 // records a failed pivot is skipped by every later launch.
 #include <algorithm>
+#include <climits>
 
 #include "llh_common.h"
 #include "k_diag_impl.h"
@@ -202,12 +203,11 @@ __global__ __launch_bounds__(256) void grp_bwd_kernel(const BatchBase<T>* __rest
 //                      64-column chunks i left of k (L's block row k, read
 //                      along its rows, the transpose taken in LDS).
 // Thread (r, c) owns row (or column) r of the 64-block and columns c, c + 4.
-// solve_system runs it for system `it` with the caller's LDS: Bs holds
-// kSolveLdsMaxN x kSolveCols values, Ts a 64 x 65 tile.
-constexpr int kSolveCols = 8;
+// solve_system runs it for system `it` (B's rows ldb apart) with the caller's
+// LDS: Bs holds kSolveLdsMaxN x kSolveCols values, Ts a 64 x 65 tile.
 template <typename T>
 __device__ __forceinline__ void solve_system(const BatchBase<T>& bb, const BatchItem& it, T* __restrict__ Bs,
-                                             T* __restrict__ Ts) {
+                                             T* __restrict__ Ts, int64_t ldb) {
   constexpr int MC = kSolveCols, S = kNB + 1;
   const int tid = threadIdx.x, n = it.n, m = it.m, nb = (n + kNB - 1) / kNB;
   const int64_t lda = it.lda;
@@ -237,7 +237,7 @@ __device__ __forceinline__ void solve_system(const BatchBase<T>& bb, const Batch
     __syncthreads();
     for (int e = tid; e < nb * kNB * MC; e += 256) {
       const int row = e / MC, cc = e % MC;
-      Bs[e] = (row < n && cc < mc) ? bm[(int64_t)row * m + c0 + cc] : T(0);
+      Bs[e] = (row < n && cc < mc) ? bm[(int64_t)row * ldb + c0 + cc] : T(0);
     }
     // forward: tile (kb, i0), i0 < 0 the inverse of block kb, else rows i0.. below it
     {
@@ -359,7 +359,7 @@ __device__ __forceinline__ void solve_system(const BatchBase<T>& bb, const Batch
     __syncthreads();
     for (int e = tid; e < n * MC; e += 256) {
       const int row = e / MC, cc = e % MC;
-      if (cc < mc) bm[(int64_t)row * m + c0 + cc] = Bs[e];
+      if (cc < mc) bm[(int64_t)row * ldb + c0 + cc] = Bs[e];
     }
   }
 }
@@ -373,7 +373,20 @@ __global__ __launch_bounds__(256) void grp_solve_kernel(const BatchBase<T>* __re
   if (failed(bb, it)) return;
   __shared__ T Bs[kSolveLdsMaxN * kSolveCols];
   __shared__ T Ts[kNB * (kNB + 1)];
-  solve_system<T>(bb, it, Bs, Ts);
+  solve_system<T>(bb, it, Bs, Ts, it.m);
+}
+
+// One system's substitutions (launch_solve_lds): one workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void solve_lds_kernel(int n, int m, const T* __restrict__ a, int64_t lda,
+                                                        const T* __restrict__ linv, T* b, int64_t ldb,
+                                                        const int32_t* __restrict__ gate) {
+  if (gate != nullptr && *gate != kNoFail) return;
+  __shared__ T Bs[kSolveLdsMaxN * kSolveCols];
+  __shared__ T Ts[kNB * (kNB + 1)];
+  const BatchBase<T> bb{const_cast<T*>(a), b, const_cast<T*>(linv), nullptr, nullptr};
+  const BatchItem it{0, 0, 0, n, m, (int32_t)lda, 0};
+  solve_system<T>(bb, it, Bs, Ts, ldb);
 }
 
 // The phases of the fused per-system kernel run one after another, so they
@@ -440,33 +453,47 @@ __global__ __launch_bounds__(256, 2) void med_kernel(const BatchBase<T>* __restr
                                                   n0, as, bs);
       __syncthreads();  // the next diagonal block is updated
     }
-    if (SOLVE && ok) solve_system<T>(bb, it, sm.solve.bs, sm.solve.ts);
+    if (SOLVE && ok) solve_system<T>(bb, it, sm.solve.bs, sm.solve.ts, it.m);
   }
 }
 
-// Elementwise steps of the mixed-precision batch (pointers read from the device
-// base, so a replayed graph follows the caller's buffers).
+// Elementwise steps of the mixed-precision batch before the factor (pointers
+// read from the device base, so a replayed graph follows the caller's buffers).
 __global__ void mixed_flat_kernel(const MixedBase* __restrict__ base, int op, int64_t elems) {
   const MixedBase mb = *base;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < elems;
        i += (int64_t)gridDim.x * blockDim.x) {
-    switch (op) {
-      case 0: mb.a32[i] = (float)mb.a64[i]; break;
-      case 1: {
-        const double v = mb.b64[i];
-        mb.x32[i] = (float)v;
-        mb.bk64[i] = v;
-        break;
-      }
-      case 2: mb.b64[i] = (double)mb.x32[i]; break;
-      case 3: mb.x32[i] = (float)mb.r64[i]; break;
-      default: mb.b64[i] += (double)mb.x32[i]; break;
+    if (op == 0) {
+      mb.a32[i] = (float)mb.a64[i];
+    } else {
+      const double v = mb.b64[i];
+      mb.x32[i] = (float)v;
+      mb.bk64[i] = v;
     }
   }
 }
 
+// b64 = x32 (add false) or b64 += x32 (add true) for rows [r0, r0 + 64) of one
+// system; a system whose factor failed keeps the caller's B.
+__global__ __launch_bounds__(256) void grp_update_kernel(const MixedBase* __restrict__ base,
+                                                         const BatchItem* __restrict__ items,
+                                                         const StepWork* __restrict__ work, int count, bool add) {
+  if ((int)blockIdx.x >= count) return;
+  const StepWork w = work[blockIdx.x];
+  const BatchItem it = items[w.item];
+  const MixedBase mb = *base;
+  if (mb.status[it.id] != kNoFail) return;
+  const int m = it.m, rows = min(64, it.n - w.k0);
+  const int64_t o = it.b_off + (int64_t)w.k0 * m;
+  for (int e = threadIdx.x; e < rows * m; e += 256) {
+    const double v = (double)mb.x32[o + e];
+    mb.b64[o + e] = add ? mb.b64[o + e] + v : v;
+  }
+}
+
 // r = bk - A x for rows [r0, r0 + 64) of one system, in f64, from the lower
-// triangle of A (A[i][k] for k > i is read as A[k][i]).  Thread (row, q) takes
+// triangle of A (A[i][k] for k > i is read as A[k][i]), stored as f32 into x32
+// (the right-hand side of the correction solve).  Thread (row, q) takes
 // the columns q, q + 4, ... of its row, two at a time.
 __global__ __launch_bounds__(256) void grp_residual_kernel(const MixedBase* __restrict__ base,
                                                            const BatchItem* __restrict__ items,
@@ -481,7 +508,7 @@ __global__ __launch_bounds__(256) void grp_residual_kernel(const MixedBase* __re
   const double* a = mb.a64 + it.a_off;
   const double* x = mb.b64 + it.b_off;
   const double* b = mb.bk64 + it.b_off;
-  double* r = mb.r64 + it.b_off;
+  float* r = mb.x32 + it.b_off;
   const int m = it.m;
   for (int c0 = q; c0 < m; c0 += 8) {
     const int c1 = c0 + 4;
@@ -491,8 +518,8 @@ __global__ __launch_bounds__(256) void grp_residual_kernel(const MixedBase* __re
       s0 += av * x[(int64_t)k * m + c0];
       if (c1 < m) s1 += av * x[(int64_t)k * m + c1];
     }
-    r[(int64_t)row * m + c0] = b[(int64_t)row * m + c0] - s0;
-    if (c1 < m) r[(int64_t)row * m + c1] = b[(int64_t)row * m + c1] - s1;
+    r[(int64_t)row * m + c0] = (float)(b[(int64_t)row * m + c0] - s0);
+    if (c1 < m) r[(int64_t)row * m + c1] = (float)(b[(int64_t)row * m + c1] - s1);
   }
 }
 
@@ -531,6 +558,19 @@ template hipError_t launch_grp_solve<double>(const BatchBase<double>*, const Bat
 template hipError_t launch_grp_solve<float>(const BatchBase<float>*, const BatchItem*, int, hipStream_t);
 
 template <typename T>
+hipError_t launch_solve_lds(int n, int m, const T* a, int64_t lda, const T* linv, T* b, int64_t ldb,
+                            const int32_t* gate, hipStream_t s) {
+  if (n <= 0 || m <= 0) return hipSuccess;
+  if (n > kSolveLdsMaxN || lda > INT32_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((solve_lds_kernel<T>), dim3(1), dim3(256), 0, s, n, m, a, lda, linv, b, ldb, gate);
+  return hipGetLastError();
+}
+template hipError_t launch_solve_lds<double>(int, int, const double*, int64_t, const double*, double*, int64_t,
+                                             const int32_t*, hipStream_t);
+template hipError_t launch_solve_lds<float>(int, int, const float*, int64_t, const float*, float*, int64_t,
+                                            const int32_t*, hipStream_t);
+
+template <typename T>
 int med_resident_per_cu(bool solve) {
   int nb = 0;
   const hipError_t e = solve ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, med_kernel<T, true>, 256, 0)
@@ -566,6 +606,13 @@ hipError_t launch_grp_residual(const MixedBase* mb, const BatchItem* items, cons
                                hipStream_t s) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(grp_residual_kernel, dim3(count), dim3(256), 0, s, mb, items, work, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_grp_update(const MixedBase* mb, const BatchItem* items, const StepWork* work, int count, bool add,
+                             hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(grp_update_kernel, dim3(count), dim3(256), 0, s, mb, items, work, count, add);
   return hipGetLastError();
 }
 

@@ -64,6 +64,15 @@ hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t ld
                         const T* dinv, bool dinv_t, T* dinv_work, SolveSync* sync, hipStream_t s,
                         Profiler* prof = nullptr, const int32_t* gate = nullptr);
 
+// x := A^-1 b with the factor L of A (lower triangle of a, ld lda) and dinv,
+// the inverses of L's 64 x 64 diagonal blocks: forward with L, back with L^T,
+// B row-major n x m (ldb) in place.  Small systems take one single-workgroup
+// launch, larger ones the two one-launch triangles of trsm_enqueue (sync).
+template <typename T>
+hipError_t solve_pair_enqueue(int n, const T* a, int64_t lda, int m, T* b, int64_t ldb, const T* dinv,
+                              SolveSync* sync, hipStream_t s, Profiler* prof = nullptr,
+                              const int32_t* gate = nullptr);
+
 // Enqueues A^-1 = L^-T L^-1 from the factor L (lower triangle of a) and the
 // inverses of its 64 x 64 diagonal blocks (dinv, as potrf_enqueue leaves them):
 // W = L^-1 into w, then W^T W into x (both triangles).  tmp holds

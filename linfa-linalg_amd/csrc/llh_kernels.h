@@ -34,15 +34,14 @@ struct BatchBase {
   int32_t* counters;  // work-queue counters of the persistent kernels
 };
 // Buffers of a batched FP32-factor / f64-refinement solve: the caller's f64 A
-// and B (B becomes X), the plan's f32 copy of A (then its factor) and f32
-// right-hand side / correction, and f64 copies of B and of the residual.
+// and B (B becomes X), the plan's f32 copy of A (then its factor), the f32
+// right-hand side / residual / correction, and the f64 copy of B.
 struct MixedBase {
   const double* a64;
   double* b64;
   float* a32;
   float* x32;
   double* bk64;
-  double* r64;
   int32_t* status;
 };
 struct BatchItem {
@@ -110,8 +109,17 @@ hipError_t launch_grp_bwd(const BatchBase<T>* base, const BatchItem* items, cons
                           hipStream_t s);
 hipError_t launch_fill_status(int32_t* status, int64_t count, hipStream_t s);
 // k_batch.hip: forward and back substitution of items[0..count), one workgroup
-// per system, right-hand sides in LDS; every n <= kSolveLdsMaxN.
+// per system, right-hand sides in LDS (kSolveCols columns at a time); every
+// n <= kSolveLdsMaxN.
 constexpr int kSolveLdsMaxN = 512;
+constexpr int kSolveCols = 8;
+// The same substitution for one system: L in the lower triangle of a (lda),
+// inverses of its 64 x 64 diagonal blocks in linv (64 x 64 each, in order), B
+// row-major n x m (ldb), n <= kSolveLdsMaxN; a no-op when *gate (if given)
+// records a failed factor.
+template <typename T>
+hipError_t launch_solve_lds(int n, int m, const T* a, int64_t lda, const T* linv, T* b, int64_t ldb,
+                            const int32_t* gate, hipStream_t s);
 template <typename T>
 hipError_t launch_grp_solve(const BatchBase<T>* base, const BatchItem* items, int count, hipStream_t s);
 // The medium bucket fused per system (k_batch.hip): factor (and with solve, the
@@ -141,13 +149,17 @@ hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha,
                        const T* B, int64_t ldb, T beta, T* C, int64_t ldc, const int32_t* gate,
                        hipStream_t s);
 
-// k_batch.hip: elementwise steps of the batched mixed-precision solve over the
-// packed buffers (op: 0 a32 = a64; 1 x32 = b64, bk64 = b64; 2 b64 = x32;
-// 3 x32 = r64; 4 b64 += x32), and the f64 residual r = bk - A x from the lower
-// triangle of each A, one workgroup per (system, 64-row block).
+// k_batch.hip: the steps of the batched mixed-precision solve.  Before the
+// factor, over the packed buffers: op 0 a32 = a64, op 1 x32 = b64 and
+// bk64 = b64.  Per (system, 64-row block), systems whose factor failed left
+// alone: the f64 residual r = bk - A x from the lower triangle of each A into
+// x32 (as f32), and the update b64 = x32 (add false, after the first solve) or
+// b64 += x32 (add true, after a correction).
 hipError_t launch_mixed_flat(const MixedBase* mb, int op, int64_t elems, hipStream_t s);
 hipError_t launch_grp_residual(const MixedBase* mb, const BatchItem* items, const StepWork* work, int count,
                                hipStream_t s);
+hipError_t launch_grp_update(const MixedBase* mb, const BatchItem* items, const StepWork* work, int count, bool add,
+                             hipStream_t s);
 
 // k_gram.hip: C (ka x kb, ldc) = A^T B for tall row-major A (n x ka) and B
 // (n x kb), split over row chunks; partial holds gram_partial_elems(...) elements.

@@ -145,10 +145,7 @@ hipError_t solve_low(llh_context* ctx, const llh_factor* f, int m, T* d, SolveSy
   const int n = f->n;
   T* l = static_cast<T*>(f->l);
   T* dinv = static_cast<T*>(f->dinv);
-  hipError_t e = trsm_enqueue<T>(false, false, n, l, n, m, d, m, dinv, false, nullptr, sync, ctx->stream,
-                                 ctx->profiler());
-  if (e != hipSuccess) return e;
-  return trsm_enqueue<T>(true, true, n, l, n, m, d, m, dinv, true, nullptr, sync, ctx->stream, ctx->profiler());
+  return solve_pair_enqueue<T>(n, l, n, m, d, m, dinv, sync, ctx->stream, ctx->profiler());
 }
 
 llh_status solve_core(llh_context* ctx, llh_factor* f, size_t m_, const double* b, size_t ldb, double* x,

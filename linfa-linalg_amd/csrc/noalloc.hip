@@ -107,12 +107,9 @@ extern "C" llh_status llh_dev_solvec_noalloc_d(llh_context* ctx, size_t n, doubl
   NA_TRY(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, s), "status");
   if (m > 0) {
     // gated on the factor's status word, as llh_dev_solvec_d
-    NA_TRY(trsm_enqueue<double>(false, false, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, false,
-                                nullptr, &sync, s, nullptr, ctx->d_info),
-           "forward solve");
-    NA_TRY(trsm_enqueue<double>(true, true, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, true, nullptr,
-                                &sync, s, nullptr, ctx->d_info),
-           "back solve");
+    NA_TRY(solve_pair_enqueue<double>((int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, &sync, s, nullptr,
+                                      ctx->d_info),
+           "solve");
   }
   NA_TRY(hipMemcpyAsync(ctx->h_info + 2, sync.words + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s),
          "solve status");

@@ -237,9 +237,7 @@ template <typename T>
 hipError_t solve_with(StreamState& st, const Entry* e, int m, T* d) {
   T* l = static_cast<T*>(e->l);
   T* dinv = static_cast<T*>(e->dinv);
-  hipError_t r = trsm_enqueue<T>(false, false, e->n, l, e->n, m, d, m, dinv, false, nullptr, &st.sync, st.s);
-  if (r != hipSuccess) return r;
-  return trsm_enqueue<T>(true, true, e->n, l, e->n, m, d, m, dinv, true, nullptr, &st.sync, st.s);
+  return solve_pair_enqueue<T>(e->n, l, e->n, m, d, m, dinv, &st.sync, st.s);
 }
 
 // Enqueues the factor of a new entry on stream st (scratch: n doubles).  The

@@ -188,6 +188,8 @@ def test_batch_f32_refined_failure_is_isolated():
     plan.solve(ta, tb, st)
     assert list(st.cpu().numpy()) == [-1, 177, 40, -1, -1]
     _, xs = unpack(plan, a, tb.cpu().numpy(), mats, bs)
+    for i in (1, 2):  # a failed factor leaves B as the caller passed it, bit for bit
+        assert np.array_equal(xs[i], bs[i]), i
     for i in (0, 3, 4):
         x_ref = orc.solvec(np.array(mats[i]), bs[i])
         assert np.abs(xs[i] - x_ref).max() <= 1e-12 * max(1, np.abs(x_ref).max())
