@@ -11,11 +11,10 @@
 // step, and no branch on the data: a failing pivot is recorded, not branched
 // on) and writes them to a row-major copy of L in LDS.  After one barrier every
 // later wave applies the step's rank-16 update to its own columns, so the
-// trailing work is spread over the four waves.  The inverse is
-// split the same way by rows: lane c holds column c of inv(L), wave w rows
-// [16w, 16w + 16); the wave owning a panel of 8 rows finishes them, publishes
-// them to a double-buffered LDS slab, and after a barrier the waves below take
-// the panel's rank-8 update.  Rows beyond b are identity rows, so the block
+// trailing work is spread over the four waves.  The inverse goes by 16-row
+// blocks: each wave inverts one 16 x 16 diagonal block, then the blocks below
+// the diagonal come from 16x16x16 MFMA products, one level of block rows at a
+// time.  Rows beyond b are identity rows, so the block
 // factors to diag(L, I) and its inverse likewise.  Pivot test, failure row and
 // write-back follow cholesky.rs:54-74: on a failure at row f the block holds
 // rows < f and L[f, ..f].
@@ -24,6 +23,7 @@
 #include <type_traits>
 #include <utility>
 
+#include "k_gemm_impl.h"
 #include "llh_common.h"
 
 namespace llh {
@@ -80,7 +80,7 @@ constexpr int kDiagT = 256;      // threads of a diagonal-block workgroup
 template <typename T>
 struct DiagLds {
   T l[kNB * kDiagS];             // L (row-major), written a panel at a time
-  T xp[2][kDiagP][kNB];          // published rows of inv(L), double-buffered by panel
+  T xd[4][16 * 17];              // inverses of the four 16 x 16 diagonal sub-blocks
   T rinv[kNB];                   // 1 / L[j, j]
   int fail[kNB / kDiagP];        // per step: failing column or -1
   int skip;                      // the system has already failed: nothing to do
@@ -211,46 +211,68 @@ __device__ __forceinline__ bool diag_quad_block(TS* __restrict__ a, int64_t lda,
     if (threadIdx.x == 0) atomicMin(info, (int32_t)(info_base + fail));
     return true;
   }
-  // inv(L): lane c solves L x = e_c; this wave holds rows [c0, c0 + 16) of x
-  const int c = i;
-  T x[WC];
+  // inv(L) by 16-row blocks.  Wave bb forms X_bb, the inverse of the 16 x 16
+  // diagonal block bb (lane c = column c, a column sweep reading L from the LDS
+  // copy as uniform-address loads).  Then, for block rows bi = 1..3, wave bj < bi
+  // forms X_ij = -X_ii sum_{k=j..i-1} L_ik X_kj as 16x16x16 MFMA products; the
+  // sum's result registers feed the second product's B operand directly (result
+  // register s holds the rows that step s takes), and X_ij goes to the free upper
+  // triangle of the LDS copy, block (j, i).  (Row panels of 8 with a barrier and a
+  // rank-8 update each took 16k of the block's 52k cycles.)
+  {
+    using MF = Mfma<T>;
+    using acc_t = typename MF::acc_t;
+    const int cb = 16 * w, cc = i & 15, fr = i & 15;
+    {
+      T x[16];
 #pragma unroll
-  for (int kk = 0; kk < WC; ++kk) x[kk] = (c0 + kk == c) ? T(1) : T(0);
-  static_for(std::make_integer_sequence<int, N / P>{}, [&](auto pi) {
-    constexpr int p = decltype(pi)::value, k0 = p * P, lo = k0 % WC;
-    if (w == p / 2) {
+      for (int q = 0; q < 16; ++q) x[q] = q == cc ? T(1) : T(0);
 #pragma unroll
-      for (int q = 0; q < P; ++q) {
-        const int k = k0 + q;
-        x[lo + q] *= sm.rinv[k];
+      for (int q = 0; q < 16; ++q) {
+        x[q] *= sm.rinv[cb + q];
 #pragma unroll
-        for (int q2 = q + 1; q2 < P; ++q2) x[lo + q2] -= l[(k0 + q2) * S + k] * x[lo + q];
+        for (int q2 = q + 1; q2 < 16; ++q2) x[q2] = fma_t(-l[(cb + q2) * S + cb + q], x[q], x[q2]);
       }
+      if (i < 16) {
 #pragma unroll
-      for (int q = 0; q < P; ++q) sm.xp[p & 1][q][c] = x[lo + q];
+        for (int q = 0; q < 16; ++q) sm.xd[w][q * 17 + cc] = x[q];
+      }
     }
     __syncthreads();
-    if (c0 + WC > k0 + P) {
-      T xq[P];
+    for (int bi = 1; bi < 4; ++bi) {
+      const int bj = w;
+      if (bj < bi) {
+        acc_t sacc = acc_t{0, 0, 0, 0};
+        for (int bk = bj; bk < bi; ++bk) {
 #pragma unroll
-      for (int q = 0; q < P; ++q) xq[q] = sm.xp[p & 1][q][c];
-#pragma unroll
-      for (int kk = 0; kk < WC; ++kk) {
-        const int ii = c0 + kk;
-        if (ii >= k0 + P) {
-          T t = x[kk];
-#pragma unroll
-          for (int q = 0; q < P; q += 2) {
-            const v2 li = *reinterpret_cast<const v2*>(&l[ii * S + k0 + q]);
-            t = fma_t(-li[0], xq[q], fma_t(-li[1], xq[q + 1], t));
+          for (int s = 0; s < 4; ++s) {
+            const int k = MF::row(i, s);
+            const T xb = bk == bj ? sm.xd[bj][k * 17 + fr] : l[(16 * bj + k) * S + 16 * bk + fr];
+            sacc = MF::mma(l[(16 * bi + fr) * S + 16 * bk + k], xb, sacc);
           }
-          x[kk] = t;
         }
-      }
-    }
-  });
+        acc_t xc = acc_t{0, 0, 0, 0};
 #pragma unroll
-  for (int kk = 0; kk < WC; ++kk) inv[(c0 + kk) * N + c] = TS(x[kk]);
+        for (int s = 0; s < 4; ++s) xc = MF::mma(-sm.xd[bi][fr * 17 + MF::row(i, s)], sacc[s], xc);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) l[(16 * bj + MF::row(i, q)) * S + 16 * bi + fr] = xc[q];
+      }
+      __syncthreads();
+    }
+  }
+  // inv, row-major 64 x 64: X_ii from xd, X_ij (i > j) from block (j, i), zero above
+#pragma unroll
+  for (int e = 0; e < N * N / kDiagT; ++e) {
+    const int idx = threadIdx.x + kDiagT * e, row = idx / N, col = idx % N;
+    const int bi = row >> 4, bj = col >> 4;
+    T v = T(0);
+    if (bi == bj) {
+      if (col <= row) v = sm.xd[bi][(row & 15) * 17 + (col & 15)];
+    } else if (bi > bj) {
+      v = l[(16 * bj + (row & 15)) * S + 16 * bi + (col & 15)];
+    }
+    inv[row * N + col] = TS(v);
+  }
   return false;
 }
 
