@@ -67,6 +67,12 @@ __global__ __launch_bounds__(256) void grp_panel_kernel(const BatchBase<T>* __re
   nt_tile<T, 64, 64, 2, 2, false, true>(below, b, b, T(1), p, it.lda, li, kNB, T(0), p, it.lda, w.m0, 0);
 }
 
+// C prefetch in the trailing tiles (f64): with k = 64 a tile has four slabs,
+// and the C tile's loads otherwise wait until after its MFMAs.  Not for f32,
+// where folding C into the sum early costs the refinement accuracy.
+template <typename T>
+constexpr bool kGrpSyrkPre = sizeof(T) == 8;
+
 // Trailing tile: A22[m0.., n0..] -= L21 L21^T (lower part).
 template <typename T>
 __global__ __launch_bounds__(256) void grp_syrk_kernel(const BatchBase<T>* __restrict__ base,
@@ -79,7 +85,8 @@ __global__ __launch_bounds__(256) void grp_syrk_kernel(const BatchBase<T>* __res
   if (failed(bb, it)) return;
   const int b = min(kNB, it.n - w.k0), m2 = it.n - w.k0 - b;
   T* p = bb.a + it.a_off + (int64_t)(w.k0 + b) * it.lda + w.k0;
-  nt_tile<T, 64, 64, 2, 2, true, true>(m2, m2, b, T(-1), p, it.lda, p, it.lda, T(1), p + b, it.lda, w.m0, w.n0);
+  nt_tile<T, 64, 64, 2, 2, true, true, kGrpSyrkPre<T>>(m2, m2, b, T(-1), p, it.lda, p, it.lda, T(1), p + b, it.lda,
+                                                        w.m0, w.n0);
 }
 
 // Forward-solve step of block k0: Y = inv(L_kk) B_k, then B_rest -= L21 Y.
@@ -449,8 +456,8 @@ __global__ __launch_bounds__(256, 2) void med_kernel(const BatchBase<T>* __restr
       __syncthreads();  // the whole panel is written before the update reads it
       for (int m0 = 0; m0 < below; m0 += 64)
         for (int n0 = 0; n0 <= m0; n0 += 64)
-          nt_tile_in<T, 64, 64, 2, 2, true, true>(below, below, b, T(-1), p, lda, p, lda, T(1), p + b, lda, m0,
-                                                  n0, as, bs);
+          nt_tile_in<T, 64, 64, 2, 2, true, true, kGrpSyrkPre<T>>(below, below, b, T(-1), p, lda, p, lda, T(1),
+                                                                  p + b, lda, m0, n0, as, bs);
       __syncthreads();  // the next diagonal block is updated
     }
     if (SOLVE && ok) solve_system<T>(bb, it, sm.solve.bs, sm.solve.ts, it.m);
