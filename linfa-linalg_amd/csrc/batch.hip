@@ -50,14 +50,16 @@ struct llh_batch_plan {
   struct SmallLaunch {
     int first, count, nmax, grid, counter;
   };
-  struct Step {
-    int k0;
-    int diag_off, ndiag;    // StepWork range (diag, trtri, fwd)
-    int panel_off, npanel;  // TileWork ranges
-    int syrk_off, nsyrk;
+  // The grouped blocked factor as a sequence of launches over work lists:
+  // diagonal blocks (StepWork, also the forward substitution's blocks), panel
+  // tiles and trailing tiles (TileWork).
+  enum PhaseKind { kDiagPhase, kPanelPhase, kSyrkPhase };
+  struct Phase {
+    PhaseKind kind;
+    int off, count;
   };
   std::vector<SmallLaunch> small;
-  std::vector<Step> steps;
+  std::vector<Phase> phases;
   std::vector<std::pair<int, int>> bwd;  // StepWork ranges of the back-solve steps
   int res_off = 0, nres = 0;             // StepWork range of the residual blocks (refined mode)
   int nclasses = 0;                      // small-system launches per pass
@@ -150,15 +152,28 @@ hipError_t record_pass(llh_batch_plan* p, hipStream_t s, int pass) {
     if ((e = hipEventRecord(p->join, p->side)) != hipSuccess) return e;
     return hipStreamWaitEvent(s, p->join, 0);
   }
-  for (const auto& st : p->steps) {
-    if (pass == 0) {
-      if ((e = launch_grp_diag_fast<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
-      if ((e = launch_grp_panel<T>(base, items, tw + st.panel_off, st.npanel, s)) != hipSuccess) return e;
+  // a diagonal-block phase is followed by that block's panel phase; the
+  // forward substitution of a block (multi-launch solves) comes after both
+  const StepWork* fwd_pending = nullptr;
+  int fwd_count = 0;
+  for (const auto& ph : p->phases) {
+    switch (ph.kind) {
+      case llh_batch_plan::kDiagPhase:
+        if (pass == 0 && (e = launch_grp_diag_fast<T>(base, items, sw + ph.off, ph.count, s)) != hipSuccess)
+          return e;
+        fwd_pending = sw + ph.off;
+        fwd_count = ph.count;
+        break;
+      case llh_batch_plan::kPanelPhase:
+        if (pass == 0 && (e = launch_grp_panel<T>(base, items, tw + ph.off, ph.count, s)) != hipSuccess) return e;
+        if (!p->lds_solve && fwd_pending != nullptr)
+          if ((e = launch_grp_fwd<T>(base, items, fwd_pending, fwd_count, s)) != hipSuccess) return e;
+        fwd_pending = nullptr;
+        break;
+      case llh_batch_plan::kSyrkPhase:
+        if (pass == 0 && (e = launch_grp_syrk<T>(base, items, tw + ph.off, ph.count, s)) != hipSuccess) return e;
+        break;
     }
-    if (!p->lds_solve)
-      if ((e = launch_grp_fwd<T>(base, items, sw + st.diag_off, st.ndiag, s)) != hipSuccess) return e;
-    if (pass == 0)
-      if ((e = launch_grp_syrk<T>(base, items, tw + st.syrk_off, st.nsyrk, s)) != hipSuccess) return e;
   }
   if (p->lds_solve) {
     // the whole forward and back substitution of every medium system, one
@@ -269,31 +284,44 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
   }
   std::vector<StepWork> sw;
   std::vector<TileWork> tw;
-  p->steps.clear();
+  p->phases.clear();
   p->bwd.clear();
-  for (int k0 = 0; k0 < maxn; k0 += kNB) {
-    llh_batch_plan::Step st{k0, (int)sw.size(), 0, 0, 0, 0, 0};
+  // Steps of two 64-column blocks: block k0 (diagonal block and panel), its
+  // update of block k0 + 64's columns only, block k0 + 64, then one rank-128
+  // update of the rest -- each trailing tile is read and written once per two
+  // blocks (with one rank-64 update per block, the C traffic of the trailing
+  // tiles was three quarters of theirs).
+  auto diag_phase = [&](int k0) {
+    const int off = (int)sw.size();
     for (int q = med_first; q < (int)items.size(); ++q)
       if (items[q].n > k0) sw.push_back({q, k0});
-    st.ndiag = (int)sw.size() - st.diag_off;
-    st.panel_off = (int)tw.size();
+    p->phases.push_back({llh_batch_plan::kDiagPhase, off, (int)sw.size() - off});
+  };
+  auto panel_phase = [&](int k0) {
+    const int off = (int)tw.size();
     for (int q = med_first; q < (int)items.size(); ++q) {
-      const int n = items[q].n;
-      if (n <= k0) continue;
-      const int below = n - k0 - std::min(kNB, n - k0);
-      for (int m0 = 0; m0 < below; m0 += 64) tw.push_back({q, k0, m0, 0});
+      const int below = items[q].n - k0 - kNB;
+      for (int m0 = 0; m0 < below; m0 += 64) tw.push_back({q, k0, m0, 0, kNB});
     }
-    st.npanel = (int)tw.size() - st.panel_off;
-    st.syrk_off = (int)tw.size();
+    p->phases.push_back({llh_batch_plan::kPanelPhase, off, (int)tw.size() - off});
+  };
+  auto syrk_phase = [&](int k0, int kb, bool first_column) {
+    const int off = (int)tw.size();
     for (int q = med_first; q < (int)items.size(); ++q) {
-      const int n = items[q].n;
-      if (n <= k0) continue;
-      const int m2 = n - k0 - std::min(kNB, n - k0);
+      const int m2 = items[q].n - k0 - kb;
       for (int m0 = 0; m0 < m2; m0 += 64)
-        for (int n0 = 0; n0 <= m0; n0 += 64) tw.push_back({q, k0, m0, n0});
+        for (int n0 = 0; n0 <= (first_column ? 0 : m0); n0 += 64) tw.push_back({q, k0, m0, n0, kb});
     }
-    st.nsyrk = (int)tw.size() - st.syrk_off;
-    p->steps.push_back(st);
+    p->phases.push_back({llh_batch_plan::kSyrkPhase, off, (int)tw.size() - off});
+  };
+  for (int k0 = 0; k0 < maxn; k0 += 2 * kNB) {
+    diag_phase(k0);
+    panel_phase(k0);
+    if (k0 + kNB >= maxn) break;
+    syrk_phase(k0, kNB, true);
+    diag_phase(k0 + kNB);
+    panel_phase(k0 + kNB);
+    if (k0 + 2 * kNB < maxn) syrk_phase(k0, 2 * kNB, false);
   }
   for (int k0 = ((maxn - 1) / kNB) * kNB; maxn > 0 && k0 >= 0; k0 -= kNB) {
     const int off = (int)sw.size();

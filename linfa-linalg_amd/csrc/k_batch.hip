@@ -73,7 +73,8 @@ __global__ __launch_bounds__(256) void grp_panel_kernel(const BatchBase<T>* __re
 template <typename T>
 constexpr bool kGrpSyrkPre = sizeof(T) == 8;
 
-// Trailing tile: A22[m0.., n0..] -= L21 L21^T (lower part).
+// Trailing tile: A22[m0.., n0..] -= L21 L21^T (lower part), L21 the kb columns
+// from k0 (kb = 64, or 128 for the pair of blocks of one step).
 template <typename T>
 __global__ __launch_bounds__(256) void grp_syrk_kernel(const BatchBase<T>* __restrict__ base,
                                                        const BatchItem* __restrict__ items,
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(256) void grp_syrk_kernel(const BatchBase<T>* __res
   const BatchItem it = items[w.item];
   const BatchBase<T> bb = *base;
   if (failed(bb, it)) return;
-  const int b = min(kNB, it.n - w.k0), m2 = it.n - w.k0 - b;
+  const int b = w.kb, m2 = it.n - w.k0 - b;
   T* p = bb.a + it.a_off + (int64_t)(w.k0 + b) * it.lda + w.k0;
   nt_tile<T, 64, 64, 2, 2, true, true, kGrpSyrkPre<T>>(m2, m2, b, T(-1), p, it.lda, p, it.lda, T(1), p + b, it.lda,
                                                         w.m0, w.n0);
@@ -441,24 +442,38 @@ __global__ __launch_bounds__(256, 2) void med_kernel(const BatchBase<T>* __restr
     const int64_t lda = it.lda;
     T* a = bb.a + it.a_off;
     bool ok = true;
-    for (int k0 = 0; k0 < n; k0 += kNB) {
+    // the diagonal block at k0, then the panel below it, as one grouped step
+    auto block = [&](int k0) -> bool {
       const int b = min(kNB, n - k0);
       T* li = bb.linv + it.linv_off + (int64_t)(k0 / kNB) * kNB * kNB;
-      if (diag_quad_block<T, T>(a + (int64_t)k0 * lda + k0, lda, b, li, bb.status + it.id, k0, sm.diag)) {
-        ok = false;  // uniform: every thread read the same failing column
-        break;
-      }
+      if (diag_quad_block<T, T>(a + (int64_t)k0 * lda + k0, lda, b, li, bb.status + it.id, k0, sm.diag))
+        return false;  // uniform: every thread read the same failing column
       __syncthreads();  // L_kk and its inverse written; the LDS is free for the tiles
       const int below = n - k0 - b;
       T* p = a + (int64_t)(k0 + b) * lda + k0;
       for (int m0 = 0; m0 < below; m0 += 64)
         nt_tile_in<T, 64, 64, 2, 2, false, true>(below, b, b, T(1), p, lda, li, kNB, T(0), p, lda, m0, 0, as, bs);
-      __syncthreads();  // the whole panel is written before the update reads it
-      for (int m0 = 0; m0 < below; m0 += 64)
-        for (int n0 = 0; n0 <= m0; n0 += 64)
-          nt_tile_in<T, 64, 64, 2, 2, true, true, kGrpSyrkPre<T>>(below, below, b, T(-1), p, lda, p, lda, T(1),
-                                                                  p + b, lda, m0, n0, as, bs);
-      __syncthreads();  // the next diagonal block is updated
+      __syncthreads();  // the whole panel is written before an update reads it
+      return true;
+    };
+    // trailing tiles (tile column 0 only, or all) of the rows below [k0, k0 + kb)
+    auto update = [&](int k0, int kb, bool first_column) {
+      const int m2 = n - k0 - kb;
+      T* p = a + (int64_t)(k0 + kb) * lda + k0;
+      for (int m0 = 0; m0 < m2; m0 += 64)
+        for (int n0 = 0; n0 <= (first_column ? 0 : m0); n0 += 64)
+          nt_tile_in<T, 64, 64, 2, 2, true, true, kGrpSyrkPre<T>>(m2, m2, kb, T(-1), p, lda, p, lda, T(1), p + kb,
+                                                                  lda, m0, n0, as, bs);
+      __syncthreads();
+    };
+    for (int k0 = 0; k0 < n && ok; k0 += 2 * kNB) {
+      // the grouped schedule's step: block k0, its update of the next block's
+      // columns, block k0 + 64, and one rank-128 update of the rest
+      if (!(ok = block(k0))) break;
+      if (k0 + kNB >= n) break;
+      update(k0, kNB, true);
+      if (!(ok = block(k0 + kNB))) break;
+      if (k0 + 2 * kNB < n) update(k0, 2 * kNB, false);
     }
     if (SOLVE && ok) solve_system<T>(bb, it, sm.solve.bs, sm.solve.ts, it.m);
   }

@@ -64,6 +64,7 @@ struct TileWork {
   int32_t k0;
   int32_t m0;
   int32_t n0;
+  int32_t kb;  // trailing tiles: width of the factored panel [k0, k0 + kb) that updates them
 };
 
 template <typename T>
