@@ -1,6 +1,7 @@
 // Batched solves of many SPD systems of uneven sizes (include/llh.h:
 // llh_batch_*): the planner, the captured hipGraph and the autotuner.
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cstring>
 #include <numeric>
@@ -286,11 +287,12 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
   std::vector<TileWork> tw;
   p->phases.clear();
   p->bwd.clear();
-  // Steps of two 64-column blocks: block k0 (diagonal block and panel), its
-  // update of block k0 + 64's columns only, block k0 + 64, then one rank-128
-  // update of the rest -- each trailing tile is read and written once per two
-  // blocks (with one rank-64 update per block, the C traffic of the trailing
-  // tiles was three quarters of theirs).
+  // Panels of kBatchPanel columns: 64-column blocks (diagonal block and panel
+  // tiles), each followed by the update of the next blocks' columns with the
+  // recursive-halving rank, and at the panel's end one update of the rest of
+  // that rank -- each trailing tile is read and written once per panel (with
+  // one rank-64 update per block, the C traffic of the trailing tiles was three
+  // quarters of theirs).
   auto diag_phase = [&](int k0) {
     const int off = (int)sw.size();
     for (int q = med_first; q < (int)items.size(); ++q)
@@ -305,24 +307,30 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
     }
     p->phases.push_back({llh_batch_plan::kPanelPhase, off, (int)tw.size() - off});
   };
-  auto syrk_phase = [&](int k0, int kb, bool first_column) {
+  auto syrk_phase = [&](int k0, int kb, int max_cols) {
     const int off = (int)tw.size();
     for (int q = med_first; q < (int)items.size(); ++q) {
       const int m2 = items[q].n - k0 - kb;
       for (int m0 = 0; m0 < m2; m0 += 64)
-        for (int n0 = 0; n0 <= (first_column ? 0 : m0); n0 += 64) tw.push_back({q, k0, m0, n0, kb});
+        for (int n0 = 0; n0 <= m0 && n0 < max_cols; n0 += 64) tw.push_back({q, k0, m0, n0, kb});
     }
     p->phases.push_back({llh_batch_plan::kSyrkPhase, off, (int)tw.size() - off});
   };
-  for (int k0 = 0; k0 < maxn; k0 += 2 * kNB) {
-    diag_phase(k0);
-    panel_phase(k0);
-    if (k0 + kNB >= maxn) break;
-    syrk_phase(k0, kNB, true);
-    diag_phase(k0 + kNB);
-    panel_phase(k0 + kNB);
-    if (k0 + 2 * kNB < maxn) syrk_phase(k0, 2 * kNB, false);
-  }
+  constexpr int nbp = kBatchPanel / kNB;
+  for (int k0 = 0; k0 < maxn; k0 += kBatchPanel)
+    for (int j = 0; j < nbp; ++j) {
+      const int kj = k0 + kNB * j;
+      if (kj >= maxn) break;
+      diag_phase(kj);
+      panel_phase(kj);
+      if (kj + kNB >= maxn) break;
+      if (j + 1 == nbp) {
+        syrk_phase(k0, kBatchPanel, INT_MAX);
+      } else {
+        const int r = lowbit(j + 1);
+        syrk_phase(k0 + kNB * (j + 1 - r), kNB * r, kNB * std::min(r, nbp - j - 1));
+      }
+    }
   for (int k0 = ((maxn - 1) / kNB) * kNB; maxn > 0 && k0 >= 0; k0 -= kNB) {
     const int off = (int)sw.size();
     for (int q = med_first; q < (int)items.size(); ++q)

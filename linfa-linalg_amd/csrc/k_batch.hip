@@ -456,25 +456,32 @@ __global__ __launch_bounds__(256, 2) void med_kernel(const BatchBase<T>* __restr
       __syncthreads();  // the whole panel is written before an update reads it
       return true;
     };
-    // trailing tiles (tile column 0 only, or all) of the rows below [k0, k0 + kb)
-    auto update = [&](int k0, int kb, bool first_column) {
+    // trailing tiles of the rows below the panel [k0, k0 + kb), in the first
+    // max_cols columns
+    auto update = [&](int k0, int kb, int max_cols) {
       const int m2 = n - k0 - kb;
       T* p = a + (int64_t)(k0 + kb) * lda + k0;
       for (int m0 = 0; m0 < m2; m0 += 64)
-        for (int n0 = 0; n0 <= (first_column ? 0 : m0); n0 += 64)
+        for (int n0 = 0; n0 <= m0 && n0 < max_cols; n0 += 64)
           nt_tile_in<T, 64, 64, 2, 2, true, true, kGrpSyrkPre<T>>(m2, m2, kb, T(-1), p, lda, p, lda, T(1), p + kb,
                                                                   lda, m0, n0, as, bs);
       __syncthreads();
     };
-    for (int k0 = 0; k0 < n && ok; k0 += 2 * kNB) {
-      // the grouped schedule's step: block k0, its update of the next block's
-      // columns, block k0 + 64, and one rank-128 update of the rest
-      if (!(ok = block(k0))) break;
-      if (k0 + kNB >= n) break;
-      update(k0, kNB, true);
-      if (!(ok = block(k0 + kNB))) break;
-      if (k0 + 2 * kNB < n) update(k0, 2 * kNB, false);
-    }
+    // the grouped schedule's order (kBatchPanel)
+    constexpr int nbp = kBatchPanel / kNB;
+    for (int k0 = 0; k0 < n && ok; k0 += kBatchPanel)
+      for (int j = 0; j < nbp; ++j) {
+        const int kj = k0 + kNB * j;
+        if (kj >= n) break;
+        if (!(ok = block(kj))) break;
+        if (kj + kNB >= n) break;
+        if (j + 1 == nbp) {
+          update(k0, kBatchPanel, INT_MAX);
+        } else {
+          const int r = lowbit(j + 1);
+          update(k0 + kNB * (j + 1 - r), kNB * r, kNB * min(r, nbp - j - 1));
+        }
+      }
     if (SOLVE && ok) solve_system<T>(bb, it, sm.solve.bs, sm.solve.ts, it.m);
   }
 }

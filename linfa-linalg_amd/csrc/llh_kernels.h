@@ -114,6 +114,14 @@ hipError_t launch_fill_status(int32_t* status, int64_t count, hipStream_t s);
 // n <= kSolveLdsMaxN.
 constexpr int kSolveLdsMaxN = 512;
 constexpr int kSolveCols = 8;
+// Panel width of the batched blocked factor (grouped launches and the fused
+// per-system kernel alike): within a panel, after its j-th 64-column block the
+// last lowbit(j + 1) blocks update the columns of as many next blocks; at the
+// panel's end one update of that rank covers the rest (recursive halving, as
+// the single-matrix factor's panels).  Measured on the 20000-system batch:
+// 7.98 / 7.47 / 7.46 / 7.56 ms with panels of 64 / 128 / 256 / 512 columns.
+constexpr int kBatchPanel = 256;
+__host__ __device__ constexpr int lowbit(int x) { return x & -x; }
 // The same substitution for one system: L in the lower triangle of a (lda),
 // inverses of its 64 x 64 diagonal blocks in linv (64 x 64 each, in order), B
 // row-major n x m (ldb), n <= kSolveLdsMaxN; a no-op when *gate (if given)
