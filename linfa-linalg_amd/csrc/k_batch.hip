@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void grp_bwd_kernel(const BatchBase<T>* __rest
 //   backward, block k: X_k = inv(L_kk)^T Y_k, then B_i -= L_ki^T X_k for the
 //                      64-column chunks i left of k (L's block row k, read
 //                      along its rows, the transpose taken in LDS).
-// Thread (r, c) owns row (or column) r of the 64-block and columns c, c + 4.
+// The 64 x 64 by 64 x 8 tile products run on MFMA (tile_mul below).
 // solve_system runs it for system `it` (B's rows ldb apart) with the caller's
 // LDS: Bs holds kSolveLdsMaxN x kSolveCols values, Ts a 64 x 65 tile.
 template <typename T>
@@ -222,8 +222,27 @@ __device__ __forceinline__ void solve_system(const BatchBase<T>& bb, const Batch
   const T* a = bb.a + it.a_off;
   const T* linv = bb.linv + it.linv_off;
   T* bm = bb.b + it.b_off;
-  const int r = tid >> 2, c = tid & 3;
   T pre[kNB * kNB / 256];
+  // The tile products on MFMA: wave w forms rows [16w, 16w + 16) of op(Ts) Y,
+  // Y the 64 x 8 right-hand-side block at y (LDS, padded to 16 columns with
+  // zeros), op(Ts) = Ts or its transpose; result register q holds row
+  // 16w + MF::row(lane, q), column lane % 16.  (One thread per output pair with
+  // scalar FMAs read three LDS values per two FMAs: 393 KB of LDS reads per
+  // tile against 64 KB here.)
+  using MF = Mfma<T>;
+  using acc_t = typename MF::acc_t;
+  const int lane = tid & 63, w16 = (tid >> 6) * 16, fr = lane & 15, fk = lane >> 4;
+  auto tile_mul = [&](const T* y, bool trans) -> acc_t {
+    acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < kNB / 4; ++s) {
+      const int k = 4 * s + fk;
+      const T av = trans ? Ts[k * S + w16 + fr] : Ts[(w16 + fr) * S + k];
+      const T bv = fr < MC ? y[k * MC + fr] : T(0);
+      acc = MF::mma(av, bv, acc);
+    }
+    return acc;
+  };
   // a tile: rows x cols of src (ld), zero outside; element e = tid + 256 g is
   // (row e / 64, column e % 64), so each wave reads one row segment
   auto fetch = [&](const T* src, int64_t ld, int rows, int cols) {
@@ -277,28 +296,19 @@ __device__ __forceinline__ void solve_system(const BatchBase<T>& bb, const Batch
         const bool more = advance(nkb, ni0);
         if (more) src(nkb, ni0);
         const int k0 = kb * kNB, b = min(kNB, n - k0);
-        const T* yk = Bs + (int64_t)k0 * MC;
-        T s0 = T(0), s1 = T(0);
+        const acc_t prod = tile_mul(Bs + (int64_t)k0 * MC, false);
         if (i0 < 0) {
-          for (int k = 0; k <= r && k < b; ++k) {
-            const T l = Ts[r * S + k];
-            s0 += l * yk[k * MC + c];
-            s1 += l * yk[k * MC + c + 4];
-          }
-          __syncthreads();
-          if (r < b) {
-            Bs[(k0 + r) * MC + c] = s0;
-            Bs[(k0 + r) * MC + c + 4] = s1;
+          __syncthreads();  // every wave has read Y_k before it is overwritten
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int row = w16 + MF::row(lane, q);
+            if (row < b && fr < MC) Bs[(k0 + row) * MC + fr] = prod[q];
           }
         } else {
-          for (int k = 0; k < b; ++k) {
-            const T l = Ts[r * S + k];
-            s0 += l * yk[k * MC + c];
-            s1 += l * yk[k * MC + c + 4];
-          }
-          if (i0 + r < n) {
-            Bs[(i0 + r) * MC + c] -= s0;
-            Bs[(i0 + r) * MC + c + 4] -= s1;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int row = w16 + MF::row(lane, q);
+            if (i0 + row < n && fr < MC) Bs[(i0 + row) * MC + fr] -= prod[q];
           }
         }
         if (!more) break;
@@ -337,27 +347,20 @@ __device__ __forceinline__ void solve_system(const BatchBase<T>& bb, const Batch
         const bool more = advance(nkb, ni0);
         if (more) src(nkb, ni0);
         const int k0 = kb * kNB, b = min(kNB, n - k0);
-        const T* xk = Bs + (int64_t)k0 * MC;
-        T s0 = T(0), s1 = T(0);
+        const acc_t prod = tile_mul(Bs + (int64_t)k0 * MC, true);
         if (i0 < 0) {
-          for (int k = r; k < b; ++k) {
-            const T l = Ts[k * S + r];
-            s0 += l * xk[k * MC + c];
-            s1 += l * xk[k * MC + c + 4];
-          }
-          __syncthreads();
-          if (r < b) {
-            Bs[(k0 + r) * MC + c] = s0;
-            Bs[(k0 + r) * MC + c + 4] = s1;
+          __syncthreads();  // every wave has read X_k before it is overwritten
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int row = w16 + MF::row(lane, q);
+            if (row < b && fr < MC) Bs[(k0 + row) * MC + fr] = prod[q];
           }
         } else {
-          for (int k = 0; k < b; ++k) {
-            const T l = Ts[k * S + r];
-            s0 += l * xk[k * MC + c];
-            s1 += l * xk[k * MC + c + 4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int row = w16 + MF::row(lane, q);
+            if (fr < MC) Bs[(i0 + row) * MC + fr] -= prod[q];
           }
-          Bs[(i0 + r) * MC + c] -= s0;
-          Bs[(i0 + r) * MC + c + 4] -= s1;
         }
         if (!more) break;
         kb = nkb;
