@@ -245,12 +245,11 @@ hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t ld
 // and the inverses of its diagonal blocks (dinv, as the factor leaves them).
 // Up to kSolvePairLdsMaxN rows and kSolveCols right-hand sides it is one
 // single-workgroup launch with B in LDS (launch_solve_lds), otherwise the two
-// one-launch triangles (trsm_enqueue).  Measured for one right-hand side, both
-// triangles: n = 129 / 192 / 256 / 384 / 512 take 165 / 186 / 255 / 420 / 607 us
-// per solvec in one workgroup against 199 / 204 / 253 / 364 / 466 us with the
-// triangles, whose flag hops (~7 us per 64 rows) cost less than one workgroup's
-// pass over L beyond 256 rows.
-constexpr int kSolvePairLdsMaxN = 256;
+// one-launch triangles (trsm_enqueue).  Measured for one right-hand side (whole
+// solvec): n = 256 / 384 / 512 take 201 / 327 / 461 us with one workgroup
+// against 201 / 348 / 445 us with the triangles, whose flag hops (~7 us per 64
+// rows) cost less than one workgroup's pass over L beyond about 450 rows.
+constexpr int kSolvePairLdsMaxN = 448;
 template <typename T>
 hipError_t solve_pair_enqueue(int n, const T* a, int64_t lda, int m, T* b, int64_t ldb, const T* dinv,
                               SolveSync* sync, hipStream_t s, Profiler* prof, const int32_t* gate) {
