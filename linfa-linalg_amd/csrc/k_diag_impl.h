@@ -77,10 +77,14 @@ constexpr int kDiagP = 8;        // panel width
 constexpr int kDiagW = 16;       // columns (factor) / rows (inverse) per wave
 constexpr int kDiagT = 256;      // threads of a diagonal-block workgroup
 
+// Packed lower triangle of a 16 x 16 block: entry (q, c), q >= c.
+__device__ __forceinline__ int xd_at(int q, int c) { return ((q * (q + 1)) >> 1) + c; }
+
+// 38.6 KB in f64 (four workgroups per CU).
 template <typename T>
 struct DiagLds {
   T l[kNB * kDiagS];             // L (row-major), written a panel at a time
-  T xd[4][16 * 17];              // inverses of the four 16 x 16 diagonal sub-blocks
+  T xd[4][137];                  // inverses of the four 16 x 16 diagonal sub-blocks, packed (xd_at), + a sink
   T rinv[kNB];                   // 1 / L[j, j]
   int fail[kNB / kDiagP];        // per step: failing column or -1
   int skip;                      // the system has already failed: nothing to do
@@ -235,7 +239,7 @@ __device__ __forceinline__ bool diag_quad_block(TS* __restrict__ a, int64_t lda,
       }
       if (i < 16) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) sm.xd[w][q * 17 + cc] = x[q];
+        for (int q = 0; q < 16; ++q) sm.xd[w][q >= cc ? xd_at(q, cc) : 136] = x[q];  // 136: a sink for q < cc
       }
     }
     __syncthreads();
@@ -247,13 +251,17 @@ __device__ __forceinline__ bool diag_quad_block(TS* __restrict__ a, int64_t lda,
 #pragma unroll
           for (int s = 0; s < 4; ++s) {
             const int k = MF::row(i, s);
-            const T xb = bk == bj ? sm.xd[bj][k * 17 + fr] : l[(16 * bj + k) * S + 16 * bk + fr];
+            const T xb = bk == bj ? (k >= fr ? sm.xd[bj][xd_at(k, fr)] : T(0))
+                                  : l[(16 * bj + k) * S + 16 * bk + fr];
             sacc = MF::mma(l[(16 * bi + fr) * S + 16 * bk + k], xb, sacc);
           }
         }
         acc_t xc = acc_t{0, 0, 0, 0};
 #pragma unroll
-        for (int s = 0; s < 4; ++s) xc = MF::mma(-sm.xd[bi][fr * 17 + MF::row(i, s)], sacc[s], xc);
+        for (int s = 0; s < 4; ++s) {
+          const int k = MF::row(i, s);
+          xc = MF::mma(fr >= k ? -sm.xd[bi][xd_at(fr, k)] : T(0), sacc[s], xc);
+        }
 #pragma unroll
         for (int q = 0; q < 4; ++q) l[(16 * bj + MF::row(i, q)) * S + 16 * bi + fr] = xc[q];
       }
@@ -267,7 +275,7 @@ __device__ __forceinline__ bool diag_quad_block(TS* __restrict__ a, int64_t lda,
     const int bi = row >> 4, bj = col >> 4;
     T v = T(0);
     if (bi == bj) {
-      if (col <= row) v = sm.xd[bi][(row & 15) * 17 + (col & 15)];
+      if (col <= row) v = sm.xd[bi][xd_at(row & 15, col & 15)];
     } else if (bi > bj) {
       v = l[(16 * bj + (row & 15)) * S + 16 * bi + (col & 15)];
     }
