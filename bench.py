@@ -600,6 +600,28 @@ def pmc_summary(pmc_dir, out_md, command, title):
     print("\n".join(lines))
 
 
+def trace_summary(trace_dir, out_md, command, title):
+    """Summarises one `rocprofv3 --kernel-trace --stats` run of this benchmark:
+    per-kernel calls, average and total duration (top 24 by total time)."""
+    import csv
+    import glob
+    hits = glob.glob(os.path.join(trace_dir, "**", "*kernel_stats.csv"), recursive=True)
+    if not hits:
+        raise SystemExit(f"no kernel_stats.csv under {trace_dir}")
+    stats = list(csv.DictReader(open(hits[0])))
+    lines = [f"# rocprofv3 summary: {title}", "", "Command (run from the repository root on one MI355X):", "",
+             "```", command, "```", "", "| kernel | calls | avg us | total ms | % |", "|---|---:|---:|---:|---:|"]
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:24]:
+        short = r["Name"].replace("void ", "").replace("(anonymous namespace)::", "").split("(")[0]
+        if len(short) > 95:
+            short = short[:92] + "..."
+        lines.append(f"| `{short}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.1f} | {float(r['Percentage']):.1f} |")
+    with open(out_md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 BATCHED_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "batched_uneven_20000_traffic.json")
 
 
@@ -676,9 +698,12 @@ def load_batched_traffic(cnt, config):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("count") != cnt or t.get("config") != config:
+    # the small bucket's resident workgroups change its speed, not the bytes it moves
+    keys = ("small_limit", "ir_iterations", "medium_mode")
+    if t.get("count") != cnt or any(t.get("config", {}).get(k) != config.get(k) for k in keys):
         return None
     return {"bytes": round(t["traffic_bytes"]), "over_algorithmic": round(t["traffic_over_algorithmic"], 3),
+            "config_profiled": t["config"],
             "source": "profiles/batched_uneven_20000.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, one launch)"}
 
 
@@ -722,9 +747,15 @@ def main():
     ap.add_argument("--batch-once", default="", metavar="SMALL_LIMIT,SMALL_WAVES[,MEDIUM_MODE]",
                     help="run one f64 graph launch of the batched workload with this configuration, print "
                          "its JSON and exit (for counter runs)")
+    ap.add_argument("--trace-summary", nargs=3, metavar=("TRACE_DIR", "OUT_MD", "TITLE"),
+                    help="summarise a rocprofv3 --kernel-trace --stats run into OUT_MD and exit")
     ap.add_argument("--batched-summary", nargs=4, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR", "ONCE_JSON"),
                     help="summarise rocprofv3 runs of --batch-once into profiles/ and exit")
     args = ap.parse_args()
+
+    if args.trace_summary:
+        trace_summary(*args.trace_summary[:2], args.rocprof_command, args.trace_summary[2])
+        return
 
     if args.batched_summary:
         batched_summary(*args.batched_summary, args.rocprof_command)
