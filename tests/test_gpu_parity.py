@@ -446,3 +446,32 @@ def test_solve_triangular_system_matches_oracle(dtype, tol, n):
         x_ref = orc.solve_triangular_system(a, np.array(b), oracle_uplo, d, True)
         assert np.all(np.isfinite(x))
         assert np.abs(x - x_ref).max() <= tol * max(1.0, np.abs(x_ref).max()), (n, uplo)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-4)])
+@pytest.mark.parametrize("n", [129, 448, 449])
+@pytest.mark.parametrize("m", [1, 8, 9])
+def test_dev_solvec_solve_pair_forms(n, m, dtype, tol):
+    """solvec's substitutions run in one workgroup with B in LDS up to 448 rows
+    and 8 right-hand sides, else as the two one-launch triangles: both give the
+    oracle's x, B with rows wider than m (strided) keeps its padding, and after a
+    failed factor B is untouched (cholesky.rs:140)."""
+    import torch
+    from linfa_linalg_hip import device as dev
+    a = spd(n, 11 * n + m, dtype)
+    b = np.random.default_rng(n + m).standard_normal((n, m)).astype(dtype)
+    tb_wide = torch.full((n, m + 3), 7.0, dtype=torch.float64 if dtype == np.float64 else torch.float32,
+                         device="cuda")
+    tb_wide[:, :m] = torch.from_numpy(b).cuda()
+    ta = torch.from_numpy(a).cuda()
+    dev.solvec(ta, tb_wide[:, :m])
+    x = tb_wide[:, :m].cpu().numpy().astype(np.float64)
+    x_ref = orc.solvec(np.array(a, dtype=np.float64), np.array(b, dtype=np.float64))
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= tol, (n, m)
+    assert torch.all(tb_wide[:, m:] == 7.0)
+    bad = np.array(a)
+    bad[n - 2, n - 2] = -1.0
+    tb2 = torch.from_numpy(b).cuda()
+    with pytest.raises(llh.NotPositiveDefinite):
+        dev.solvec(torch.from_numpy(bad).cuda(), tb2)
+    assert np.array_equal(tb2.cpu().numpy(), b)
