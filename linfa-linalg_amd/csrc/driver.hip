@@ -244,12 +244,11 @@ hipError_t trsm_enqueue(bool upper_op, bool trans, int n, const T* a, int64_t ld
 // The forward and back substitution with the factor L (lower triangle of a)
 // and the inverses of its diagonal blocks (dinv, as the factor leaves them).
 // Up to kSolvePairLdsMaxN rows and kSolveCols right-hand sides it is one
-// single-workgroup launch with B in LDS (launch_solve_lds), otherwise the two
-// one-launch triangles (trsm_enqueue).  Measured for one right-hand side (whole
-// solvec): n = 256 / 384 / 512 take 201 / 327 / 461 us with one workgroup
-// against 201 / 348 / 445 us with the triangles, whose flag hops (~7 us per 64
-// rows) cost less than one workgroup's pass over L beyond about 450 rows.
-constexpr int kSolvePairLdsMaxN = 448;
+// single-workgroup launch with B in LDS and L streamed through registers
+// (launch_solve_lds), otherwise the two one-launch triangles (trsm_enqueue),
+// whose flag hops cost ~7 us per 64 rows.  For n = 512 and one right-hand side
+// the one-workgroup launch takes 68 us against 2 x 57 us for the triangles.
+constexpr int kSolvePairLdsMaxN = kSolveLdsMaxN;
 template <typename T>
 hipError_t solve_pair_enqueue(int n, const T* a, int64_t lda, int m, T* b, int64_t ldb, const T* dinv,
                               SolveSync* sync, hipStream_t s, Profiler* prof, const int32_t* gate) {

@@ -375,6 +375,199 @@ __device__ __forceinline__ void solve_system(const BatchBase<T>& bb, const Batch
   }
 }
 
+// A workgroup barrier that orders LDS accesses only: __syncthreads() also
+// waits for every global load in flight (one counter covers loads and stores),
+// which would drain the tile prefetch at every diagonal block.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The same two sweeps for one system with L streamed (solve_lds_kernel).  The
+// 64 x 64 tiles come in a fixed order -- forward: per block k the inverse, then
+// the row chunks below; backward: per block k the transposed inverse, then the
+// column chunks to its left -- and no load depends on a result, so three tiles
+// are in flight at any time.  Each wave works on its own 16 rows of a tile
+// (16 rows of L, or 16 columns for a transposed tile): it loads them with
+// consecutive lanes on consecutive addresses, stages them in its own slice of
+// LDS and reads them back as lane (fr, fk) = (row fr, k values 16fk .. 16fk +
+// 15), so no workgroup barrier sits between tiles; the MCT partial sums of a
+// row are reduced over the four fk lanes by two xor shuffles.  Only the
+// diagonal blocks synchronise the workgroup (all of B_k is read before Y_k
+// replaces it).  VALU dot products: with MCT <= 8 columns an MFMA tile would be
+// at least half padding.  (Loading the dot layout straight from global memory
+// puts 64 cache lines behind every load instruction: 2.0k against ~0.7k cycles
+// per tile.)
+template <typename T>
+constexpr int kStageElems = sizeof(T) == 8 ? 64 * 18 : 64 * 20;  // per wave
+
+template <typename T, int MCT, bool VEC>
+__device__ __forceinline__ void solve_stream(int n, int m, const T* __restrict__ a, int64_t lda,
+                                             const T* __restrict__ linv, T* b, int64_t ldb, T* __restrict__ Bs,
+                                             T* __restrict__ stage) {
+  constexpr int KS = 16, VL = 16 / sizeof(T), NQ = KS / VL;
+  constexpr int SR = kNB + VL, SC = KS + VL;  // staged row strides: 16 x 64 rows, 64 x 16 columns
+  static_assert(16 * SR <= kStageElems<T> && kNB * SC <= kStageElems<T>, "a staged tile fits the slice");
+  typedef T vec_t __attribute__((ext_vector_type(VL)));
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, fr = lane & 15, fk = lane >> 4;
+  const int r = 16 * w + fr, kq = KS * fk;
+  T* const ws = stage + w * kStageElems<T>;
+  const int nb = (n + kNB - 1) / kNB, per = nb * (nb + 1) / 2, total = 2 * per;
+  // tile t: block kb and chunk i (-1: the diagonal block's inverse)
+  auto decode = [&](int t, int& kb, int& i) {
+    if (t < per) {
+      for (int k = 0;; ++k) {
+        if (t < nb - k) {
+          kb = k;
+          i = t == 0 ? -1 : k + t;
+          return;
+        }
+        t -= nb - k;
+      }
+    }
+    t -= per;
+    for (int k = nb - 1;; --k) {
+      if (t < k + 1) {
+        kb = k;
+        i = t - 1;
+        return;
+      }
+      t -= k + 1;
+    }
+  };
+  // VEC: a, lda and linv keep VL-element groups 16-byte aligned (else scalar
+  // loads; a run-time choice per load would put a branch join after every
+  // load, where the compiler waits for all of them)
+  auto load_vl = [&](const T* q) -> vec_t {
+    if constexpr (VEC) {
+      return *reinterpret_cast<const vec_t*>(q);
+    } else {
+      vec_t x;
+#pragma unroll
+      for (int e = 0; e < VL; ++e) x[e] = q[e];
+      return x;
+    }
+  };
+  // Global loads only, no masking (a value that depends on a lane's bounds
+  // test makes the compiler wait for the load where it is issued).  Rows
+  // beyond n are read from row n - 1 (finite values of L's lower triangle) and
+  // come to nothing: they meet rows of B that stay zero, or their sums are not
+  // written.  A tile is either the wave's 16 rows of 64 columns (rows) or all
+  // 64 rows of the wave's 16 columns: lane positions for both, chosen per tile
+  // by a select, so that every fetch is the same straight-line run of NQ loads
+  // (with a branch per tile kind the compiler cannot tell which loads a later
+  // use waits for, and waits for all of them).
+  constexpr int LR = kNB / VL, LC = KS / VL;  // lanes per staged row: 64 or 16 columns
+  const int row_r = 16 * w + lane / LR, col_r = VL * (lane % LR);
+  const int row_c = lane / LC, col_c = 16 * w + VL * (lane % LC);
+  auto fetch = [&](int t, vec_t* g) {
+    int kb, i;
+    decode(min(t, total - 1), kb, i);  // past the end: the last tile again (not used)
+    const bool rows = t < per;
+    const int k0 = kb * kNB;
+    const T* p;
+    int64_t ld;
+    int rmax;
+    if (i < 0) {
+      p = linv + (int64_t)kb * kNB * kNB;
+      ld = kNB;
+      rmax = kNB - 1;
+    } else if (rows) {
+      p = a + (int64_t)(i * kNB) * lda + k0;
+      ld = lda;
+      rmax = n - 1 - i * kNB;
+    } else {
+      p = a + (int64_t)k0 * lda + i * kNB;
+      ld = lda;
+      rmax = n - 1 - k0;
+    }
+    const int row0 = rows ? row_r : row_c, dq = rows ? VL : kNB / LC, col = rows ? col_r : col_c;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) g[q] = load_vl(p + (int64_t)min(row0 + q * dq, rmax) * ld + col);
+  };
+  auto process = [&](int t, const vec_t* g) {
+    int kb, i;
+    decode(t, kb, i);
+    const bool rows = t < per;
+    // stage the wave's part of the tile, then read it as (row fr, k = kq ..)
+    T v[KS];
+    {
+      const int base = rows ? (lane / LR) * SR + col_r : (lane / LC) * SC + (col_c - 16 * w);
+      const int step = rows ? VL * SR : (kNB / LC) * SC;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) *reinterpret_cast<vec_t*>(ws + base + q * step) = g[q];
+    }
+    if (rows) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const vec_t x = *reinterpret_cast<const vec_t*>(ws + fr * SR + kq + q * VL);
+#pragma unroll
+        for (int e = 0; e < VL; ++e) v[q * VL + e] = x[e];
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < KS; ++s) v[s] = ws[(kq + s) * SC + fr];
+    }
+    const T* x = Bs + (kb * kNB + kq) * MCT;
+    if (i < 0) lds_barrier();  // every wave's updates of B_k are done
+    T acc[MCT];
+#pragma unroll
+    for (int c = 0; c < MCT; ++c) acc[c] = T(0);
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int c = 0; c < MCT; ++c) acc[c] = fma_t(v[s], x[s * MCT + c], acc[c]);
+#pragma unroll
+    for (int c = 0; c < MCT; ++c) {
+      acc[c] += __shfl_xor(acc[c], 16);
+      acc[c] += __shfl_xor(acc[c], 32);
+    }
+    if (i < 0) {
+      lds_barrier();  // every wave has read B_k
+#pragma unroll
+      for (int c = 0; c < MCT; ++c)
+        if ((c & 3) == fk && kb * kNB + r < n) Bs[(kb * kNB + r) * MCT + c] = acc[c];
+      lds_barrier();  // Y_k (X_k) is complete
+    } else {
+#pragma unroll
+      for (int c = 0; c < MCT; ++c)
+        if ((c & 3) == fk && i * kNB + r < n) Bs[(i * kNB + r) * MCT + c] -= acc[c];
+    }
+  };
+  for (int c0 = 0; c0 < m; c0 += MCT) {
+    const int mc = min(MCT, m - c0);
+    lds_barrier();  // the previous chunk's results are stored
+    for (int e = tid; e < nb * kNB * MCT; e += 256) {
+      const int row = e / MCT, cc = e % MCT;
+      Bs[e] = (row < n && cc < mc) ? b[(int64_t)row * ldb + c0 + cc] : T(0);
+    }
+    // process t while t + 1 and t + 2 are in flight (the scheduling barriers
+    // keep the compiler from sinking the loads to their use)
+    vec_t g0[NQ], g1[NQ], g2[NQ];
+    fetch(0, g0);
+    fetch(1, g1);
+    for (int t = 0; t < total; t += 3) {
+      fetch(t + 2, g2);
+      __builtin_amdgcn_sched_barrier(0);
+      process(t, g0);
+      if (t + 1 >= total) break;
+      fetch(t + 3, g0);
+      __builtin_amdgcn_sched_barrier(0);
+      process(t + 1, g1);
+      if (t + 2 >= total) break;
+      fetch(t + 4, g1);
+      __builtin_amdgcn_sched_barrier(0);
+      process(t + 2, g2);
+    }
+    lds_barrier();
+    for (int e = tid; e < n * MCT; e += 256) {
+      const int row = e / MCT, cc = e % MCT;
+      if (cc < mc) b[(int64_t)row * ldb + c0 + cc] = Bs[e];
+    }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void grp_solve_kernel(const BatchBase<T>* __restrict__ base,
                                                         const BatchItem* __restrict__ items, int count) {
@@ -387,17 +580,16 @@ __global__ __launch_bounds__(256) void grp_solve_kernel(const BatchBase<T>* __re
   solve_system<T>(bb, it, Bs, Ts, it.m);
 }
 
-// One system's substitutions (launch_solve_lds): one workgroup.
-template <typename T>
+// One system's substitutions (launch_solve_lds): one workgroup, MCT columns of
+// B at a time.
+template <typename T, int MCT, bool VEC>
 __global__ __launch_bounds__(256) void solve_lds_kernel(int n, int m, const T* __restrict__ a, int64_t lda,
                                                         const T* __restrict__ linv, T* b, int64_t ldb,
                                                         const int32_t* __restrict__ gate) {
   if (gate != nullptr && *gate != kNoFail) return;
-  __shared__ T Bs[kSolveLdsMaxN * kSolveCols];
-  __shared__ T Ts[kNB * (kNB + 1)];
-  const BatchBase<T> bb{const_cast<T*>(a), b, const_cast<T*>(linv), nullptr, nullptr};
-  const BatchItem it{0, 0, 0, n, m, (int32_t)lda, 0};
-  solve_system<T>(bb, it, Bs, Ts, ldb);
+  __shared__ __attribute__((aligned(16))) T Bs[kSolveLdsMaxN * MCT];
+  __shared__ __attribute__((aligned(16))) T stage[4 * kStageElems<T>];
+  solve_stream<T, MCT, VEC>(n, m, a, lda, linv, b, ldb, Bs, stage);
 }
 
 // The phases of the fused per-system kernel run one after another, so they
@@ -594,7 +786,19 @@ hipError_t launch_solve_lds(int n, int m, const T* a, int64_t lda, const T* linv
                             const int32_t* gate, hipStream_t s) {
   if (n <= 0 || m <= 0) return hipSuccess;
   if (n > kSolveLdsMaxN || lda > INT32_MAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((solve_lds_kernel<T>), dim3(1), dim3(256), 0, s, n, m, a, lda, linv, b, ldb, gate);
+  const bool vec = (uintptr_t)a % 16 == 0 && (lda * (int64_t)sizeof(T)) % 16 == 0 && (uintptr_t)linv % 16 == 0;
+  auto go = [&](auto mct, auto vc) {
+    hipLaunchKernelGGL((solve_lds_kernel<T, decltype(mct)::value, decltype(vc)::value>), dim3(1), dim3(256), 0, s,
+                       n, m, a, lda, linv, b, ldb, gate);
+  };
+  auto pick = [&](auto vc) {
+    if (m == 1) go(std::integral_constant<int, 1>{}, vc);
+    else if (m == 2) go(std::integral_constant<int, 2>{}, vc);
+    else if (m <= 4) go(std::integral_constant<int, 4>{}, vc);
+    else go(std::integral_constant<int, kSolveCols>{}, vc);
+  };
+  if (vec) pick(std::true_type{});
+  else pick(std::false_type{});
   return hipGetLastError();
 }
 template hipError_t launch_solve_lds<double>(int, int, const double*, int64_t, const double*, double*, int64_t,

@@ -116,7 +116,7 @@ def _server_case(srv, mid, a, b, prec="f64"):
 
 def test_server_request_reports_a_wait_that_gave_up():
     from linfa_linalg_hip import device as dev
-    n = 500  # above the batched small-system bound: one-launch sweeps
+    n = 600  # above the one-workgroup substitution's 512 rows: one-launch sweeps
     a = spd(n, 10)
     b = np.random.default_rng(11).standard_normal((n, 4))
     x_ref = orc.solvec(np.array(a), b)

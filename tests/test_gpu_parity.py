@@ -449,11 +449,12 @@ def test_solve_triangular_system_matches_oracle(dtype, tol, n):
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-4)])
-@pytest.mark.parametrize("n", [129, 448, 449])
-@pytest.mark.parametrize("m", [1, 8, 9])
+@pytest.mark.parametrize("n", [129, 200, 448, 512, 513])
+@pytest.mark.parametrize("m", [1, 2, 3, 8, 9])
 def test_dev_solvec_solve_pair_forms(n, m, dtype, tol):
-    """solvec's substitutions run in one workgroup with B in LDS up to 448 rows
-    and 8 right-hand sides, else as the two one-launch triangles: both give the
+    """solvec's substitutions run in one workgroup with B in LDS up to 512 rows
+    and 8 right-hand sides (1, 2, 4 or 8 columns per pass; rows of L 16-byte
+    aligned or not), else as the two one-launch triangles: both give the
     oracle's x, B with rows wider than m (strided) keeps its padding, and after a
     failed factor B is untouched (cholesky.rs:140)."""
     import torch
