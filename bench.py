@@ -9,8 +9,9 @@ the GPU before the timed region; every step factors its own copy of A, so the
 timed region contains only the solve.
 
 GFLOP/s counts the algorithmic work n^3/3 + n^2/2 + n/6 + 2 n^2 m.  The
-dominant kernel (the rank-256 trailing SYRK on MFMA) is timed live with HIP
-events on the launch stream; its roofline fraction is against the FP64 peak.
+dominant kernel (the trailing SYRK on MFMA, of rank 1024 at n >= 24576) is
+timed live with HIP events on the launch stream; its roofline fraction is
+against the FP64 peak.
 The CPU baseline is the oracle (the reference algorithm restated in C, one
 thread) on a bounded sample of the same workload.
 """
