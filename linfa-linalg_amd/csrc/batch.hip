@@ -68,6 +68,8 @@ struct llh_batch_plan {
   bool lds_solve = false;                // every medium n <= kSolveLdsMaxN: one substitution launch
   bool med_fused = false;                // the medium bucket's factor runs as one fused per-system launch
   bool med_fused_solve = false;          // ... and its substitutions in the same launch
+  bool fused_fwd = false;                // grouped medium bucket: the factor's steps carry the forward sweep
+  void* d_y = nullptr;                   // ... into this workspace (B's layout)
   int med_counter = 0, med_grid = 0;     // its work-queue counter and resident grid
   int ncounters = 0;
   size_t linv_elems = 0;
@@ -124,6 +126,8 @@ void free_schedule(llh_batch_plan* p) {
   }
   if (p->d_counters) (void)hipFree(p->d_counters);
   p->d_counters = nullptr;
+  if (p->d_y) (void)hipFree(p->d_y);
+  p->d_y = nullptr;
 }
 
 // One pass over the batch: pass 0 factors and solves; a later pass (the
@@ -277,6 +281,12 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
   p->lds_solve = maxn <= kSolveLdsMaxN;
   p->med_fused = p->lds_solve && p->nmed > 0 && p->cfg.medium_mode != 2;
   p->med_fused_solve = p->med_fused && p->cfg.medium_mode == 0;
+  // Grouped steps with one substitution launch: each diagonal step also forms
+  // Y_k = inv(L_kk) Y_k and each panel tile Y_i -= L_ik Y_k from the tile it
+  // has just written, so the substitution launch reads L once (back sweep
+  // only).  Same products in the same order as its forward sweep.  Not in the
+  // refined mode, whose solve-only passes have no factor steps.
+  p->fused_fwd = p->lds_solve && p->nmed > 0 && p->cfg.medium_mode == 2 && !refined(p);
   p->med_counter = p->ncounters;
   if (p->med_fused) {
     p->ncounters += 1;
@@ -353,6 +363,7 @@ llh_status build_schedule(llh_context* ctx, llh_batch_plan* p, llh_error* err) {
   CHECK_HIP(upload(&p->d_step, sw.data(), sw.size() * sizeof(StepWork)), "batch work");
   CHECK_HIP(upload(&p->d_tile, tw.data(), tw.size() * sizeof(TileWork)), "batch tiles");
   CHECK_HIP(hipMalloc(&p->d_counters, std::max(64, p->ncounters) * sizeof(int32_t)), "batch counters");
+  if (p->fused_fwd) CHECK_HIP(hipMalloc(&p->d_y, std::max<size_t>(p->b_elems * esize(p), 64)), "batch forward");
   const size_t lb = std::max<size_t>(p->linv_elems * compute_esize(p), 64);
   if (p->linv_bytes < lb) {
     if (p->d_linv) (void)hipFree(p->d_linv);
@@ -384,7 +395,7 @@ llh_status run(llh_context* ctx, llh_batch_plan* p, void* a, void* b, int32_t* s
   // base pointers (outside the graph), then one graph launch
   if (refined(p)) {
     // host staging of the two bases lives in the plan: the copies are async
-    p->h_base32 = BatchBase<float>{p->a32, p->x32, static_cast<float*>(p->d_linv), status, p->d_counters};
+    p->h_base32 = BatchBase<float>{p->a32, p->x32, static_cast<float*>(p->d_linv), status, p->d_counters, nullptr};
     p->h_mbase = MixedBase{static_cast<const double*>(a), static_cast<double*>(b), p->a32, p->x32, p->bk64,
                            status};
     CHECK_HIP(hipMemcpyAsync(p->d_base, &p->h_base32, sizeof(p->h_base32), hipMemcpyHostToDevice, ctx->stream),
@@ -393,11 +404,11 @@ llh_status run(llh_context* ctx, llh_batch_plan* p, void* a, void* b, int32_t* s
               "batch base");
   } else if (p->prec == LLH_PREC_F32) {
     BatchBase<float> bb{static_cast<float*>(a), static_cast<float*>(b), static_cast<float*>(p->d_linv), status,
-                        p->d_counters};
+                        p->d_counters, static_cast<float*>(p->d_y)};
     CHECK_HIP(hipMemcpyAsync(p->d_base, &bb, sizeof(bb), hipMemcpyHostToDevice, ctx->stream), "batch base");
   } else {
     BatchBase<double> bb{static_cast<double*>(a), static_cast<double*>(b), static_cast<double*>(p->d_linv),
-                         status, p->d_counters};
+                         status, p->d_counters, static_cast<double*>(p->d_y)};
     CHECK_HIP(hipMemcpyAsync(p->d_base, &bb, sizeof(bb), hipMemcpyHostToDevice, ctx->stream), "batch base");
   }
   CHECK_HIP(hipGraphLaunch(p->exec, ctx->stream), "graph launch");

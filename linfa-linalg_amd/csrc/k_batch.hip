@@ -10,6 +10,7 @@
 #include "llh_common.h"
 #include "k_diag_impl.h"
 #include "k_gemm_impl.h"
+#include "k_subst_impl.h"
 #include "k_trsm_impl.h"
 #include "llh_kernels.h"
 
@@ -51,7 +52,8 @@ __global__ __launch_bounds__(256) void grp_trtri_kernel(const BatchBase<T>* __re
 }
 
 // Panel tile: L21[m0.., :] = A21[m0.., :] * inv(L11)^T, in place (the tile holds
-// all b <= 64 columns of its rows, so the rows it reads are the rows it writes).
+// all b <= 64 columns of its rows, so the rows it reads are the rows it writes);
+// with a forward workspace (bb.y), then Y of its rows -= L21 Y_k.
 template <typename T>
 __global__ __launch_bounds__(256) void grp_panel_kernel(const BatchBase<T>* __restrict__ base,
                                                         const BatchItem* __restrict__ items,
@@ -64,7 +66,27 @@ __global__ __launch_bounds__(256) void grp_panel_kernel(const BatchBase<T>* __re
   const int b = min(kNB, it.n - w.k0), below = it.n - w.k0 - b;
   T* p = bb.a + it.a_off + (int64_t)(w.k0 + b) * it.lda + w.k0;
   const T* li = bb.linv + it.linv_off + (int64_t)(w.k0 / kNB) * kNB * kNB;
-  nt_tile<T, 64, 64, 2, 2, false, true>(below, b, b, T(1), p, it.lda, li, kNB, T(0), p, it.lda, w.m0, 0);
+  // the product's slabs, then (forward step) the tile and Y_k: 37 KB in f64,
+  // four workgroups per CU either way
+  __shared__ __attribute__((aligned(16))) union {
+    struct {
+      NtSlab<T, 64> as[2], bs[2];
+    } t;
+    struct {
+      T ts[kNB * (kNB + 1)];
+      T ys[kNB * kSolveCols];
+    } f;
+  } sm;
+  if (bb.y == nullptr) {
+    nt_tile_in<T, 64, 64, 2, 2, false, true>(below, b, b, T(1), p, it.lda, li, kNB, T(0), p, it.lda, w.m0, 0,
+                                             sm.t.as, sm.t.bs);
+    return;
+  }
+  const FwdPrefetch<T> f = fwd_panel_prefetch<T>(bb, it, w.k0, w.m0);
+  nt_tile_in<T, 64, 64, 2, 2, false, true, false, T, true>(below, b, b, T(1), p, it.lda, li, kNB, T(0), p, it.lda,
+                                                          w.m0, 0, sm.t.as, sm.t.bs, sm.f.ts);
+  __syncthreads();  // the tile is in ts
+  grp_forward_panel<T>(bb, it, w.k0, w.m0, sm.f.ts, sm.f.ys, f);
 }
 
 // C prefetch in the trailing tiles (f64): with k = 64 a tile has four slabs,
@@ -210,7 +232,9 @@ __global__ __launch_bounds__(256) void grp_bwd_kernel(const BatchBase<T>* __rest
 //   backward, block k: X_k = inv(L_kk)^T Y_k, then B_i -= L_ki^T X_k for the
 //                      64-column chunks i left of k (L's block row k, read
 //                      along its rows, the transpose taken in LDS).
-// The 64 x 64 by 64 x 8 tile products run on MFMA (tile_mul below).
+// The 64 x 64 by 64 x 8 tile products run on MFMA (tile_mul).  When the
+// grouped factor's steps carried the forward sweep (a plan's workspace bb.y),
+// only the back sweep runs here, starting from Y.
 // solve_system runs it for system `it` (B's rows ldb apart) with the caller's
 // LDS: Bs holds kSolveLdsMaxN x kSolveCols values, Ts a 64 x 65 tile.
 template <typename T>
@@ -232,6 +256,8 @@ __device__ __forceinline__ void solve_system(const BatchBase<T>& bb, const Batch
   using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;
   const int lane = tid & 63, w16 = (tid >> 6) * 16, fr = lane & 15, fk = lane >> 4;
+  // (subst_tile's product, with the transposed form: the same k order, so the
+  // forward steps that the grouped factor carries give the same bits)
   auto tile_mul = [&](const T* y, bool trans) -> acc_t {
     acc_t acc = acc_t{0, 0, 0, 0};
 #pragma unroll
@@ -262,12 +288,16 @@ __device__ __forceinline__ void solve_system(const BatchBase<T>& bb, const Batch
   for (int c0 = 0; c0 < m; c0 += MC) {
     const int mc = min(MC, m - c0);
     __syncthreads();
+    // with a forward workspace the factor's steps have done the forward sweep:
+    // start from Y (B's layout, rows m apart)
+    const T* src = bb.y != nullptr ? bb.y + it.b_off : bm;
+    const int64_t ld_src = bb.y != nullptr ? (int64_t)m : ldb;
     for (int e = tid; e < nb * kNB * MC; e += 256) {
       const int row = e / MC, cc = e % MC;
-      Bs[e] = (row < n && cc < mc) ? bm[(int64_t)row * ldb + c0 + cc] : T(0);
+      Bs[e] = (row < n && cc < mc) ? src[(int64_t)row * ld_src + c0 + cc] : T(0);
     }
     // forward: tile (kb, i0), i0 < 0 the inverse of block kb, else rows i0.. below it
-    {
+    if (bb.y == nullptr) {
       auto src = [&](int kb, int i0) {
         const int k0 = kb * kNB;
         if (i0 < 0) fetch(linv + (int64_t)kb * kNB * kNB, kNB, kNB, kNB);

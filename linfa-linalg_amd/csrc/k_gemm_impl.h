@@ -68,6 +68,8 @@ constexpr int mfma_lane_k() { return sizeof(TI) == 2 ? 8 : 1; }
 //
 // Rows beyond M (or N) are clamped to the last valid row, so interior loads
 // carry no bounds checks; their products land in rows the epilogue discards.
+// With TOUT, the stored tile is also left in tile_out (LDS, row stride BN + 1,
+// free once the last slab is used: it may alias the slabs), zero outside C.
 // Only a final partial slab of k is loaded with masking.  A may alias C only when
 // each tile's rows of A are its own rows of C (every read precedes the writes).
 //
@@ -88,10 +90,12 @@ constexpr int mfma_lane_k() { return sizeof(TI) == 2 ? 8 : 1; }
 template <typename TI, int ROWS>
 using NtSlab = TI[ROWS * 9 * (16 / sizeof(TI))];
 
-template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool VEC, bool PRE = false, typename TI = T>
+template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool VEC, bool PRE = false, typename TI = T,
+          bool TOUT = false>
 __device__ __forceinline__ void nt_tile_in(int M, int N, int K, T alpha, const TI* A, int64_t lda,
                                            const TI* __restrict__ B, int64_t ldb, T beta, T* C, int64_t ldc,
-                                           int m0, int n0, NtSlab<TI, BM>* As, NtSlab<TI, BN>* Bs) {
+                                           int m0, int n0, NtSlab<TI, BM>* As, NtSlab<TI, BN>* Bs,
+                                           T* tile_out = nullptr) {
   constexpr int NTHR = WM * WN * 64;
   constexpr int TM = BM / WM, TN = BN / WN;
   constexpr int FM = TM / 16, FN = TN / 16;
@@ -246,11 +250,14 @@ __device__ __forceinline__ void nt_tile_in(int M, int N, int K, T alpha, const T
       for (int r = 0; r < 4; ++r) {
         const int gi = m0 + wm0 + i * 16 + MF::row(lane, r);
         const int gj = n0 + wn0 + j * 16 + fr;
+        T out = T(0);
         if (gi < M && gj < N && (!TRI || gi >= gj)) {
           T* c = C + (int64_t)gi * ldc + gj;
           const T v = alpha * acc[i][j][r];
-          *c = (beta == T(0) || (PRE && pre)) ? v : beta * *c + v;
+          out = (beta == T(0) || (PRE && pre)) ? v : beta * *c + v;
+          *c = out;
         }
+        if constexpr (TOUT) tile_out[(wm0 + i * 16 + MF::row(lane, r)) * (BN + 1) + wn0 + j * 16 + fr] = out;
       }
 }
 

@@ -32,6 +32,7 @@ struct BatchBase {
   T* linv;
   int32_t* status;
   int32_t* counters;  // work-queue counters of the persistent kernels
+  T* y;               // B's layout: the forward sweep, when the grouped factor's steps carry it; else null
 };
 // Buffers of a batched FP32-factor / f64-refinement solve: the caller's f64 A
 // and B (B becomes X), the plan's f32 copy of A (then its factor), the f32
