@@ -520,17 +520,18 @@ llh_status dev_solvec(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T*
   CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, false, dinv, ctx->stream, ctx->profiler(), false,
                             ctx_lookahead(ctx)),
             "potrf");
-  CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
-            "status");
   if (m > 0) {
     // The solves are enqueued at once (one synchronisation per call) but gated
     // on the factor's status word: after NotPositiveDefinite every solve kernel
     // is a no-op and B keeps the caller's values, as the reference returns
-    // before touching b (cholesky.rs:140).
+    // before touching b (cholesky.rs:140).  The status is read back after
+    // them, so its copy does not sit between the factor and the solves.
     CHECK_HIP(solve_pair_enqueue<T>((int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, &sync, ctx->stream,
                                     ctx->profiler(), ctx->d_info),
               "solve");
   }
+  CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
+            "status");
   CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
   if (ctx->h_info[0] != kNoFail) return not_pd(err, ctx->h_info[0]);

@@ -84,18 +84,22 @@ def test_batch_against_oracle(dtype, small_limit):
                 assert np.abs(x - x_ref).max() <= 1e3 * eps * max(1.0, np.abs(x_ref).max()), n
 
 
+@pytest.mark.parametrize("medium_mode", [0, 2])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_batch_lds_substitution_against_oracle(dtype):
+def test_batch_lds_substitution_against_oracle(dtype, medium_mode):
     """Every blocked system within 512 rows: the whole forward and back
     substitution of each runs in one workgroup with its right-hand sides in
-    LDS (8 columns at a time: m = 9..20 take several chunks)."""
+    LDS (8 columns at a time: m = 9..20 take several chunks).  With the grouped
+    steps (medium_mode 2) the forward sweep rides on the factor's diagonal and
+    panel launches and the substitution launch runs only the back sweep."""
     from linfa_linalg_hip import device as dev
     rng = np.random.default_rng(11)
     sizes = [129, 130, 191, 192, 193, 255, 256, 257, 400, 447, 448, 449, 511, 512] + \
         list(rng.integers(129, 513, 30))
     rhs = [0, 1, 2, 3, 5, 7, 8, 9, 12, 16, 17, 20, 1, 8] + [int(x) for x in rng.integers(1, 9, 30)]
     mats, bs = make_batch(sizes, rhs, dtype, 17)
-    plan = dev.BatchPlan(sizes, rhs, prec="f64" if dtype == np.float64 else "f32", small_limit=128)
+    plan = dev.BatchPlan(sizes, rhs, prec="f64" if dtype == np.float64 else "f32", small_limit=128,
+                         medium_mode=medium_mode)
     fac, xs, st = run(plan, mats, bs, dtype)
     assert np.all(st == -1)
     eps = np.finfo(dtype).eps
