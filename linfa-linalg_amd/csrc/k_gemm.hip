@@ -244,7 +244,9 @@ __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_ker
 
   // C prefetch for the triangular updates (the trailing SYRK and the look-ahead
   // panel's columns): f64 on 256 x 128 tiles (the 128 x 128 f64 tile has no
-  // registers left) and f16 operands on 128 x 128 tiles.  Measured on the SYRK
+  // registers left) and on 64 x 64 tiles (the small blocked factors' updates:
+  // one n = 512 solvec 403 -> 383 us, n = 16384 40.9 -> 40.3 ms), and f16
+  // operands on 128 x 128 tiles.  Measured on the SYRK
   // at n = 32256, k = 512: f64 53 -> 57, f16 199 -> 348 TFLOP/s; on the
   // look-ahead update as well, n = 32768: f64 solve 219.7 -> 217.4 ms, FP16+IR
   // 81.0 -> 79.6 ms.  Not
@@ -254,7 +256,7 @@ __global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_ker
   // the f32 accumulators with C before the first slab is no better: 119 instead
   // of 104 TFLOP/s alone, no gain inside the factor (149.6 -> 148.4 ms at
   // n = 32768), and 29 instead of 4 refinement steps at lambda = 1e-6.
-  constexpr bool PRE = TRI && ((BM == 256 && sizeof(T) == 8) || sizeof(TI) == 2);
+  constexpr bool PRE = TRI && (((BM == 256 || BM == 64) && sizeof(T) == 8) || sizeof(TI) == 2);
   nt_tile<T, BM, BN, WM, WN, TRI, VEC, PRE, TI>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0);
 }
 
