@@ -24,7 +24,7 @@ __global__ __launch_bounds__(kDiagT) void diag_fast_kernel(TS* __restrict__ a, i
 // the block at (k0, k0) of its system; a failure sets the system's status.
 // With a forward workspace (bb.y) it also takes the block's forward step.
 template <typename T>
-__global__ __launch_bounds__(kDiagT) void grp_diag_fast_kernel(const BatchBase<T>* __restrict__ base,
+__global__ __launch_bounds__(kDiagT, 4) void grp_diag_fast_kernel(const BatchBase<T>* __restrict__ base,
                                                            const BatchItem* __restrict__ items,
                                                            const StepWork* __restrict__ work, int count) {
   if ((int)blockIdx.x >= count) return;
@@ -36,14 +36,17 @@ __global__ __launch_bounds__(kDiagT) void grp_diag_fast_kernel(const BatchBase<T
   if (threadIdx.x == 0) sm.skip = *st != kNoFail;
   __syncthreads();
   if (sm.skip) return;
+  const bool fwd = bb.y != nullptr;
+  FwdDiagPrefetch<T> f{};
+  if (fwd) f = fwd_diag_prefetch<T>(bb, it, w.k0);
   if (diag_quad_block<T, T>(bb.a + it.a_off + (int64_t)w.k0 * it.lda + w.k0, it.lda, min(kNB, it.n - w.k0),
                             bb.linv + it.linv_off + (int64_t)(w.k0 / kNB) * kNB * kNB, st, w.k0, sm))
     return;
-  if (bb.y != nullptr) {
+  if (fwd) {
     __syncthreads();  // the inverse is stored; the block's LDS is free
     static_assert(sizeof(sm.l) >= kNB * (kNB + 1) * sizeof(T) && sizeof(sm.xd) >= kNB * kSolveCols * sizeof(T),
                   "the forward step's tiles fit the block's LDS");
-    grp_forward_diag<T>(bb, it, w.k0, sm.l, &sm.xd[0][0]);
+    grp_forward_diag<T>(bb, it, w.k0, sm.l, &sm.xd[0][0], f);
   }
 }
 

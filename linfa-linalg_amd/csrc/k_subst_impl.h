@@ -51,38 +51,63 @@ __device__ __forceinline__ void subst_stage(const T* __restrict__ src, int64_t l
   }
 }
 
+// Right-hand-side rows the forward steps read: Y (B's layout) from the first
+// block's steps on; those steps read B itself and write Y, so Y needs no copy.
+template <typename T>
+__device__ __forceinline__ const T* fwd_src(const BatchBase<T>& bb, const BatchItem& it, int k0) {
+  return (k0 == 0 ? bb.b : bb.y) + it.b_off;
+}
+
+// Y_k (rows k0 .., the first kSolveCols columns) loaded before the block's
+// factor, so the load overlaps it: element e = threadIdx.x + 256 g is row
+// e / 8, column e % 8, from clamped indices (masked when staged).
+template <typename T>
+struct FwdDiagPrefetch {
+  T yk[kNB * kSolveCols / 256];
+};
+template <typename T>
+__device__ __forceinline__ FwdDiagPrefetch<T> fwd_diag_prefetch(const BatchBase<T>& bb, const BatchItem& it,
+                                                                int k0) {
+  constexpr int MC = kSolveCols;
+  const int n = it.n, m = it.m, b = min(kNB, n - k0), mc = min(MC, m);
+  const T* src = fwd_src(bb, it, k0);
+  FwdDiagPrefetch<T> f;
+#pragma unroll
+  for (int g = 0; g < kNB * MC / 256; ++g) {
+    const int e = threadIdx.x + 256 * g, k = e / MC, c = e % MC;
+    f.yk[g] = mc > 0 ? src[(int64_t)(k0 + min(k, b - 1)) * m + min(c, mc - 1)] : T(0);
+  }
+  return f;
+}
+
 // Forward-substitution step of the diagonal block at k0 of system `it` in the
-// grouped factor: Y_k = inv(L_kk) Y_k, after copying B into Y at the first
-// block.  Called by all 256 threads once the block's inverse is stored; ts
-// (64 x 65) and ys (64 x kSolveCols) are LDS the caller no longer needs.  (The
-// substitution kernel's forward sweep does the same per block.)
+// grouped factor: Y_k = inv(L_kk) Y_k.  Called by all 256 threads once the
+// block's inverse is stored; ts (64 x 65) and ys (64 x kSolveCols) are LDS the
+// caller no longer needs, f what fwd_diag_prefetch loaded.  (The substitution
+// kernel's forward sweep does the same per block.)
 template <typename T>
 __device__ void grp_forward_diag(const BatchBase<T>& bb, const BatchItem& it, int k0, T* __restrict__ ts,
-                                 T* __restrict__ ys) {
+                                 T* __restrict__ ys, const FwdDiagPrefetch<T>& f) {
   using MF = Mfma<T>;
   constexpr int MC = kSolveCols;
   const int tid = threadIdx.x, n = it.n, m = it.m, b = min(kNB, n - k0);
   const int lane = tid & 63, w16 = (tid >> 6) * 16, fr = lane & 15;
-  T* y = bb.y + it.b_off;
-  if (k0 == 0) {
-    const T* src = bb.b + it.b_off;
-    for (int64_t e = tid; e < (int64_t)n * m; e += 256) y[e] = src[e];
-    __syncthreads();
-  }
+  const T* src = fwd_src(bb, it, k0) + (int64_t)k0 * m;
+  T* yk = bb.y + it.b_off + (int64_t)k0 * m;
   subst_stage<T>(bb.linv + it.linv_off + (int64_t)(k0 / kNB) * kNB * kNB, kNB, kNB, kNB, ts);
   for (int c0 = 0; c0 < m; c0 += MC) {
     const int mc = min(MC, m - c0);
-    T* yk = y + (int64_t)k0 * m + c0;
-    for (int e = tid; e < kNB * MC; e += 256) {
-      const int k = e / MC, c = e % MC;
-      ys[e] = (k < b && c < mc) ? yk[(int64_t)k * m + c] : T(0);
+#pragma unroll
+    for (int g = 0; g < kNB * MC / 256; ++g) {
+      const int e = tid + 256 * g, k = e / MC, c = e % MC;
+      ys[e] = (k < b && c < mc) ? (c0 == 0 ? f.yk[g] : src[(int64_t)k * m + c0 + c]) : T(0);
     }
     __syncthreads();
     const auto prod = subst_tile<T>(ts, ys);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int row = w16 + MF::row(lane, q);
-      if (row < b && fr < mc) yk[(int64_t)row * m + fr] = prod[q];
+      if (row < b && fr < mc) yk[(int64_t)row * m + c0 + fr] = prod[q];
     }
     __syncthreads();  // ys is restaged for the next columns
   }
@@ -91,13 +116,13 @@ __device__ void grp_forward_diag(const BatchBase<T>& bb, const BatchItem& it, in
 // Forward-substitution update of a panel tile of the grouped factor: Y rows
 // [i0, i0 + 64) -= L_ik Y_k with i0 = k0 + 64 + m0.  The caller has just
 // stored L_ik (all 64 columns: a block with rows below is full) and left it in
-// ts (row stride 65, zero below the system's last row); ys is free LDS.  yk0
-// and yi0 hold what fwd_panel_prefetch loaded for the first kSolveCols columns
-// before the tile's product, so those loads overlap it.
+// ts (row stride 65, zero below the system's last row); ys is free LDS.  f
+// holds what fwd_panel_prefetch loaded for the first kSolveCols columns before
+// the tile's product, so those loads overlap it.
 template <typename T>
 struct FwdPrefetch {
   T yk[kNB * kSolveCols / 256];  // Y_k, element e = threadIdx.x + 256 g: row e / 8, column e % 8
-  T yi[4];                       // Y_i at the rows and column this lane's product registers hold
+  T yi[4];                       // the rows this tile updates, at the rows and column of this lane's products
 };
 template <typename T>
 __device__ __forceinline__ FwdPrefetch<T> fwd_panel_prefetch(const BatchBase<T>& bb, const BatchItem& it, int k0,
@@ -107,16 +132,17 @@ __device__ __forceinline__ FwdPrefetch<T> fwd_panel_prefetch(const BatchBase<T>&
   const int tid = threadIdx.x, lane = tid & 63, w16 = (tid >> 6) * 16, fr = lane & 15;
   const int n = it.n, m = it.m, i0 = k0 + kNB + m0, rows = min(kNB, n - i0), mc = min(MC, m);
   const T* y = bb.y + it.b_off;
+  const T* src = fwd_src(bb, it, k0);
   FwdPrefetch<T> f;
 #pragma unroll
   for (int g = 0; g < kNB * MC / 256; ++g) {
     const int e = tid + 256 * g, k = e / MC, c = e % MC;
-    f.yk[g] = y[(int64_t)(k0 + k) * m + min(c, mc - 1)];
+    f.yk[g] = mc > 0 ? y[(int64_t)(k0 + k) * m + min(c, mc - 1)] : T(0);
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int row = min(w16 + MF::row(lane, q), rows - 1);
-    f.yi[q] = y[(int64_t)(i0 + row) * m + min(fr, mc - 1)];
+    f.yi[q] = mc > 0 ? src[(int64_t)(i0 + row) * m + min(fr, mc - 1)] : T(0);
   }
   return f;
 }
@@ -128,6 +154,7 @@ __device__ void grp_forward_panel(const BatchBase<T>& bb, const BatchItem& it, i
   const int tid = threadIdx.x, lane = tid & 63, w16 = (tid >> 6) * 16, fr = lane & 15;
   const int n = it.n, m = it.m, i0 = k0 + kNB + m0, rows = min(kNB, n - i0);
   T* y = bb.y + it.b_off;
+  const T* src = fwd_src(bb, it, k0);
   for (int c0 = 0; c0 < m; c0 += MC) {
     const int mc = min(MC, m - c0);
     const T* yk = y + (int64_t)k0 * m + c0;
@@ -142,8 +169,8 @@ __device__ void grp_forward_panel(const BatchBase<T>& bb, const BatchItem& it, i
     for (int q = 0; q < 4; ++q) {
       const int row = w16 + MF::row(lane, q);
       if (row < rows && fr < mc) {
-        T* yr = y + (int64_t)(i0 + row) * m + c0 + fr;
-        *yr = (c0 == 0 ? f.yi[q] : *yr) - prod[q];
+        const int64_t at = (int64_t)(i0 + row) * m + c0 + fr;
+        y[at] = (c0 == 0 ? f.yi[q] : src[at]) - prod[q];
       }
     }
     __syncthreads();  // ys is restaged for the next columns
