@@ -92,7 +92,9 @@ struct DiagLds {
 
 // TS: storage type of the block and of the inverse; T: arithmetic type.
 // Called by all kDiagT threads of a workgroup; returns true (in every thread)
-// when the block failed, after recording the failing row in *info.
+// when the block failed, after recording the failing row in *info.  inv (64 x
+// 64, row-major) is stored by column pairs: it must be 2 * sizeof(TS)-aligned
+// (every caller's inverse slots are whole 64 x 64 blocks of an allocation).
 template <typename TS, typename T>
 __device__ __forceinline__ bool diag_quad_block(TS* __restrict__ a, int64_t lda, int b, TS* __restrict__ inv,
                                                 int32_t* __restrict__ info, int64_t info_base, DiagLds<T>& sm) {
@@ -203,13 +205,28 @@ __device__ __forceinline__ bool diag_quad_block(TS* __restrict__ a, int64_t lda,
 #pragma unroll
   for (int ws = 0; ws < N / WC; ++ws) fail = fail < 0 ? sm.fail[ws] : fail;
   // write back from the LDS copy of L (every panel up to the failing one was
-  // published there), consecutive lanes on consecutive columns of a row:
-  // rows < f (or all rows) with their lower part, row f up to f-1
+  // published there), consecutive lanes on consecutive column pairs of a row
+  // (one 16-byte LDS read, and in f64 one 16-byte store when the pair is whole
+  // and a's rows keep pairs aligned): rows < f (or all rows) with their lower
+  // part, row f up to f-1
+  typedef TS s2 __attribute__((ext_vector_type(2)));
+  {
+    const bool a2 = (uintptr_t)a % (2 * sizeof(TS)) == 0 && lda % 2 == 0;
 #pragma unroll
-  for (int e = 0; e < N * N / kDiagT; ++e) {
-    const int idx = threadIdx.x + kDiagT * e, row = idx / N, col = idx % N;
-    const int last = (fail < 0 || row < fail) ? row : (row == fail ? row - 1 : -1);
-    if (row < b && col <= last) a[(int64_t)row * lda + col] = TS(l[row * S + col]);
+    for (int e = 0; e < N * N / (2 * kDiagT); ++e) {
+      const int idx = threadIdx.x + kDiagT * e, row = idx / (N / 2), col = 2 * (idx % (N / 2));
+      const int last = (fail < 0 || row < fail) ? row : (row == fail ? row - 1 : -1);
+      const v2 v = *reinterpret_cast<const v2*>(&l[row * S + col]);
+      TS* dst = a + (int64_t)row * lda + col;
+      if (row < b) {
+        if (a2 && col + 1 <= last) {
+          *reinterpret_cast<s2*>(dst) = s2{TS(v[0]), TS(v[1])};
+        } else {
+          if (col <= last) dst[0] = TS(v[0]);
+          if (col + 1 <= last) dst[1] = TS(v[1]);
+        }
+      }
+    }
   }
   if (fail >= 0) {
     if (threadIdx.x == 0) atomicMin(info, (int32_t)(info_base + fail));
@@ -268,18 +285,22 @@ __device__ __forceinline__ bool diag_quad_block(TS* __restrict__ a, int64_t lda,
       __syncthreads();
     }
   }
-  // inv, row-major 64 x 64: X_ii from xd, X_ij (i > j) from block (j, i), zero above
+  // inv, row-major 64 x 64 (column pairs, as above): X_ii from xd, X_ij (i > j)
+  // from block (j, i), zero above
 #pragma unroll
-  for (int e = 0; e < N * N / kDiagT; ++e) {
-    const int idx = threadIdx.x + kDiagT * e, row = idx / N, col = idx % N;
-    const int bi = row >> 4, bj = col >> 4;
-    T v = T(0);
+  for (int e = 0; e < N * N / (2 * kDiagT); ++e) {
+    const int idx = threadIdx.x + kDiagT * e, row = idx / (N / 2), col = 2 * (idx % (N / 2));
+    const int bi = row >> 4, bj = col >> 4, rr = row & 15, cc = col & 15;
+    T v0 = T(0), v1 = T(0);
     if (bi == bj) {
-      if (col <= row) v = sm.xd[bi][xd_at(row & 15, col & 15)];
+      if (cc <= rr) v0 = sm.xd[bi][xd_at(rr, cc)];
+      if (cc + 1 <= rr) v1 = sm.xd[bi][xd_at(rr, cc + 1)];
     } else if (bi > bj) {
-      v = l[(16 * bj + (row & 15)) * S + 16 * bi + (col & 15)];
+      const v2 u = *reinterpret_cast<const v2*>(&l[(16 * bj + rr) * S + 16 * bi + cc]);
+      v0 = u[0];
+      v1 = u[1];
     }
-    inv[row * N + col] = TS(v);
+    *reinterpret_cast<s2*>(&inv[row * N + col]) = s2{TS(v0), TS(v1)};
   }
   return false;
 }
