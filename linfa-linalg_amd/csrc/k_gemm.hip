@@ -212,7 +212,7 @@ __device__ inline void tri_tile(int lin, int tiles_m, int tiles_n, int R, int& t
 // SQ: the square triangular case (TRI and M == N), i.e. the trailing SYRK.
 // TI: operand type (T, or _Float16 for the f32 SYRK on f16 operands).
 template <typename T, int BM, int BN, int WM, int WN, bool TRI, bool SQ, bool VEC, typename TI = T>
-__global__ __launch_bounds__(WM* WN * 64, 512 / (WM * WN * 64)) void gemm_nt_kernel(int M, int N, int K, T alpha,
+__global__ __launch_bounds__(WM* WN * 64, (512 / (WM * WN * 64)) > 0 ? 512 / (WM * WN * 64) : 1) void gemm_nt_kernel(int M, int N, int K, T alpha,
                                                                  const TI* A, int64_t lda,
                                                                  const TI* __restrict__ B, int64_t ldb, T beta,
                                                                  T* C, int64_t ldc,
@@ -328,13 +328,15 @@ hipError_t launch_gemm(bool ta, bool tb, bool tri, int M, int N, int K, T alpha,
     // look-ahead panel update (M = 31744, N = K = 1024) at 5.3 ms per launch
     // against 2.0 ms for f64 on the same shapes, while 128 x 128 is the faster
     // f32 tile for the SYRK too.
+    // 128 x 128 tiles on 16 waves, as the SYRK's (f64 solve at n = 32768 215.7
+    // -> 212.8 ms, FP32+IR 140.0 -> 136.0 ms against 4 waves).
     const bool big = sizeof(T) == 8 && (int64_t)M * N >= (int64_t)4096 * 4096;
     if (tri) {
       if (big) return launch_nt<T, 256, 128, 4, 2, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
-      return launch_nt<T, 128, 128, 2, 2, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
+      return launch_nt<T, 128, 128, 4, 4, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
     }
     if (big) return launch_nt<T, 256, 128, 4, 2, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
-    return launch_nt<T, 128, 128, 2, 2, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
+    return launch_nt<T, 128, 128, 4, 4, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
   }
   if (!ta && tb && tri)
     return launch_shape<T, false, true, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, gate, s);
@@ -355,9 +357,14 @@ hipError_t launch_syrk_lower(int n, int k, T alpha, const T* A, int64_t lda, T b
   // f64: 256 x 128 tiles on 8 waves (2 per SIMD) while the grid still fills the
   // chip several times over, 128 x 128 tiles for the last few small updates.
   // f32: 128 x 128 tiles throughout (measured faster than 256 x 128 for f32).
+  // The 128 x 128 tiles run on 16 waves (32 x 32 each, 56 VGPRs in f32, eight
+  // waves per SIMD): n = 32768 FP32+IR 148.8 -> 140.0 ms (f32 SYRK 88 -> 97
+  // TFLOP/s) against the 4-wave form; 8 waves (2 x 4 or 4 x 2) gave 142 ms,
+  // 256 x 128 on 16 waves 147 ms.  For f64, 16 waves on 256 x 128 (a 64 x 32
+  // tile per wave at 128 VGPRs) took the solve from 216 to 225 ms.
   if (sizeof(T) == 8 && (int64_t)n * n >= (int64_t)4096 * 4096)
     return launch_nt<T, 256, 128, 4, 2, true, true>(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, gate, s);
-  return launch_nt<T, 128, 128, 2, 2, true, true>(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, gate, s);
+  return launch_nt<T, 128, 128, 4, 4, true, true>(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, gate, s);
 }
 
 hipError_t launch_syrk_lower_h(int n, int k, float alpha, const _Float16* A, int64_t lda, float beta, float* C,
