@@ -214,7 +214,8 @@ LLH_DEV_DECL(float, s)
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
  * semantics and results as llh_dev_solvec_d; the scratch that call takes from
  * the context's grow-only buffers (the inverses of the 64 x 64 diagonal blocks,
- * and the one-launch solve's ticket, error word and flags) lives instead in a
+ * the right-hand sides of the forward substitution fused into the factor, and
+ * the one-launch solve's ticket, error word and flags) lives instead in a
  * static device array of LLH_NOALLOC_SCRATCH_BYTES that every GPU the library
  * runs on holds until the process ends (llh_context_create locates it, once per
  * device).  The call makes no device allocation.  n > LLH_NOALLOC_MAX_N is
@@ -222,7 +223,7 @@ LLH_DEV_DECL(float, s)
  * GPU's array and the array keeps the last call's contents: calls that overlap
  * on one GPU (from any context or thread) race and must be serialized by the
  * caller. */
-#define LLH_NOALLOC_SCRATCH_BYTES (8u * 1024u * 1024u + 16u * 1024u)
+#define LLH_NOALLOC_SCRATCH_BYTES (16u * 1024u * 1024u + 16u * 1024u)
 #define LLH_NOALLOC_MAX_N 16384
 llh_status llh_dev_solvec_noalloc_d(llh_context* ctx, size_t n, double* a, size_t lda, size_t m, double* b,
                                     size_t ldb, llh_error* err);

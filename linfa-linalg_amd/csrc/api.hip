@@ -109,6 +109,15 @@ const Lookahead* ctx_lookahead(llh_context* ctx) {
       (void)hipStreamDestroy(la.side);
       return nullptr;
     }
+    // the fused forward substitution's stream is optional: without it the
+    // solves run after the factor
+    if (hipStreamCreateWithFlags(&la.fwd, hipStreamNonBlocking) != hipSuccess) {
+      la.fwd = nullptr;
+    } else if (hipEventCreateWithFlags(&la.fwd_ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipStreamDestroy(la.fwd);
+      la.fwd = nullptr;
+      la.fwd_ev = nullptr;
+    }
     ctx->la = la;
   }
   return &ctx->la;
@@ -343,6 +352,22 @@ llh_status solve_triangular_plain(llh_context* ctx, size_t rows, size_t cols, co
   return ok(err);
 }
 
+// factor_solve_enqueue on the context stream, the fused forward substitution's
+// workspace taken from the grow-only buffer fwd_y.
+template <typename T>
+hipError_t ctx_factor_solve(llh_context* ctx, int n, T* a, int64_t lda, int m, T* b, int64_t ldb, T* dinv,
+                            SolveSync* sync) {
+  const Lookahead* la = ctx_lookahead(ctx);
+  T* y = nullptr;
+  if (forward_fusable<T>(n, m, la)) {
+    bool fresh = false;
+    hipError_t e = grow(ctx, ctx->fwd_y, (size_t)n * m * sizeof(T), &fresh);
+    if (e != hipSuccess) return e;
+    y = static_cast<T*>(ctx->fwd_y.ptr);
+  }
+  return factor_solve_enqueue<T>(a, lda, n, m, b, ldb, dinv, y, sync, ctx->d_info, ctx->stream, ctx->profiler(), la);
+}
+
 // Small systems go through the exact-order fused kernel (bit-identical to the
 // reference's factor and both sweeps); larger ones through the blocked path.
 template <typename T>
@@ -360,12 +385,8 @@ llh_status solve_on_device(llh_context* ctx, size_t n, T* da, size_t m, T* db, l
     T* dinv = static_cast<T*>(dv);
     SolveSync sync;
     CHECK_HIP(ctx_sync(ctx, (int)n, (int)m, &sync), "solve sync");
-    CHECK_HIP(potrf_enqueue<T>(da, (int64_t)n, (int)n, ctx->d_info, false, dinv, ctx->stream, ctx->profiler(),
-                                               false, ctx_lookahead(ctx)),
-              "potrf");
-    CHECK_HIP(solve_pair_enqueue<T>((int)n, da, (int64_t)n, (int)m, db, (int64_t)m, dinv, &sync, ctx->stream,
-                                    ctx->profiler(), ctx->d_info),
-              "solve");
+    CHECK_HIP(ctx_factor_solve<T>(ctx, (int)n, da, (int64_t)n, (int)m, db, (int64_t)m, dinv, &sync),
+              "factor and solve");
   }
   CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
   return LLH_OK;
@@ -517,19 +538,13 @@ llh_status dev_solvec(llh_context* ctx, size_t n, T* a, size_t lda, size_t m, T*
   T* dinv = static_cast<T*>(dv);
   SolveSync sync;
   CHECK_HIP(ctx_sync(ctx, (int)n, (int)std::max<size_t>(m, 1), &sync), "solve sync");
-  CHECK_HIP(potrf_enqueue<T>(a, (int64_t)lda, (int)n, ctx->d_info, false, dinv, ctx->stream, ctx->profiler(), false,
-                            ctx_lookahead(ctx)),
-            "potrf");
-  if (m > 0) {
-    // The solves are enqueued at once (one synchronisation per call) but gated
-    // on the factor's status word: after NotPositiveDefinite every solve kernel
-    // is a no-op and B keeps the caller's values, as the reference returns
-    // before touching b (cholesky.rs:140).  The status is read back after
-    // them, so its copy does not sit between the factor and the solves.
-    CHECK_HIP(solve_pair_enqueue<T>((int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, &sync, ctx->stream,
-                                    ctx->profiler(), ctx->d_info),
-              "solve");
-  }
+  // The solves are enqueued at once (one synchronisation per call) but gated on
+  // the factor's status word: after NotPositiveDefinite every solve kernel is a
+  // no-op and B keeps the caller's values, as the reference returns before
+  // touching b (cholesky.rs:140).  The status is read back after them, so its
+  // copy does not sit between the factor and the solves.
+  CHECK_HIP(ctx_factor_solve<T>(ctx, (int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, &sync),
+            "factor and solve");
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
             "status");
   CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
@@ -727,6 +742,7 @@ void llh_context_destroy(llh_context* ctx) {
   if (ctx->mixed_scale.ptr) (void)hipFree(ctx->mixed_scale.ptr);
   if (ctx->half_panel.ptr) (void)hipFree(ctx->half_panel.ptr);
   if (ctx->sync.ptr) (void)hipFree(ctx->sync.ptr);
+  if (ctx->fwd_y.ptr) (void)hipFree(ctx->fwd_y.ptr);
   if (ctx->d_info) (void)hipFree(ctx->d_info);
   if (ctx->h_info) (void)hipHostFree(ctx->h_info);
   if (ctx->la.side) {
@@ -734,6 +750,11 @@ void llh_context_destroy(llh_context* ctx) {
     (void)hipEventDestroy(ctx->la.fork);
     (void)hipEventDestroy(ctx->la.join);
     (void)hipStreamDestroy(ctx->la.side);
+    if (ctx->la.fwd) {
+      (void)hipStreamSynchronize(ctx->la.fwd);
+      (void)hipEventDestroy(ctx->la.fwd_ev);
+      (void)hipStreamDestroy(ctx->la.fwd);
+    }
   }
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->magic = 0;

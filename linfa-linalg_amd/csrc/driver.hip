@@ -100,9 +100,27 @@ hipError_t trailing_syrk(int m, int kb, const T* p, int64_t lda, T* c, int32_t* 
 // operation order.
 template <typename T>
 hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, T* dinv, hipStream_t s,
-                         Profiler* prof, bool diag_f64, const Lookahead* la, _Float16* half) {
+                         Profiler* prof, bool diag_f64, const Lookahead* la, _Float16* half, const FwdFuse<T>* ff) {
   const double es = (double)sizeof(T);
   if (half != nullptr && sizeof(T) != 4) return hipErrorInvalidValue;
+  if (ff != nullptr && (la == nullptr || la->fwd == nullptr || dinv == nullptr || n <= small_max<T>() ||
+                        ff->m <= 0 || ff->m > 64 || ff->sync == nullptr))
+    return hipErrorInvalidValue;
+  // The fused forward step of the outer panel [k0, k0 + kb), on la->fwd once
+  // the panel is factored: Y_k := L11^-1 Y_k (one-launch triangle), then
+  // Y[k0 + kb:] -= L21 Y_k.
+  auto fwd_panel = [&](int k0, int kb) -> hipError_t {
+    const int m = ff->m;
+    T* yk = ff->y + (int64_t)k0 * ff->ldy;
+    LLH_TRY(trsm_enqueue<T>(false, false, kb, a + (int64_t)k0 * lda + k0, lda, m, yk, ff->ldy,
+                            dinv + (int64_t)(k0 / kNB) * kNB * kNB, false, nullptr, ff->sync, la->fwd, prof, info));
+    const int rest = n - k0 - kb;
+    if (rest <= 0) return hipSuccess;
+    LLH_PROF_ON(la->fwd, kSolveGemm, 2.0 * rest * m * kb, es * ((double)rest * kb + 2.0 * rest * m + (double)kb * m),
+                (launch_gemm<T>(false, false, false, rest, m, kb, T(-1), a + (int64_t)(k0 + kb) * lda + k0, lda, yk,
+                                ff->ldy, T(1), yk + (int64_t)kb * ff->ldy, ff->ldy, info, la->fwd)));
+    return hipSuccess;
+  };
   // bytes per panel element of the trailing update (the f16 copy is written
   // and read once more)
   const double pes = half != nullptr ? es + 4.0 : es;
@@ -115,6 +133,13 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
   } else {
     const int nbo = outer_width(n, half != nullptr);
     LLH_TRY(factor_panel<T>(a, lda, n, 0, std::min(nbo, n), info, dinv, s, prof, diag_f64));
+    if (ff != nullptr) {
+      LLH_TRY(hipEventRecord(la->fwd_ev, s));
+      LLH_TRY(hipStreamWaitEvent(la->fwd, la->fwd_ev, 0));
+      LLH_TRY(hipMemcpy2DAsync(ff->y, ff->ldy * sizeof(T), ff->b, ff->ldb * sizeof(T), ff->m * sizeof(T), n,
+                               hipMemcpyDeviceToDevice, la->fwd));
+      LLH_TRY(fwd_panel(0, std::min(nbo, n)));
+    }
     for (int k0 = 0; k0 < n; k0 += nbo) {
       const int kb = std::min(nbo, n - k0);
       const int m2 = n - k0 - kb;
@@ -160,6 +185,10 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
         LLH_TRY(factor_panel<T>(a, lda, n, k0 + kb, kb2, info, dinv, la->side, prof, diag_f64));
       }
       LLH_TRY(hipEventRecord(la->join, la->side));
+      if (ff != nullptr) {
+        LLH_TRY(hipStreamWaitEvent(la->fwd, la->join, 0));
+        LLH_TRY(fwd_panel(k0 + kb, kb2));
+      }
       const int m3 = m2 - kb2;  // the rest of the trailing matrix
       if (m3 > 0) {
         const double outs = (double)m3 * (m3 + 1) / 2.0;
@@ -175,6 +204,10 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
         }
       }
       LLH_TRY(hipStreamWaitEvent(s, la->join, 0));
+    }
+    if (ff != nullptr) {
+      LLH_TRY(hipEventRecord(la->fwd_ev, la->fwd));
+      LLH_TRY(hipStreamWaitEvent(s, la->fwd_ev, 0));
     }
   }
   if (zero_upper) LLH_TRY(launch_zero_triangle<T>(a, lda, n, true, info, s));
@@ -262,6 +295,36 @@ hipError_t solve_pair_enqueue(int n, const T* a, int64_t lda, int m, T* b, int64
   return trsm_enqueue<T>(true, true, n, a, lda, m, b, ldb, dinv, true, nullptr, sync, s, prof, gate);
 }
 
+template <typename T>
+bool forward_fusable(int n, int m, const Lookahead* la) {
+  return m > 0 && m <= 64 && n > outer_width(n, false) && n > kSolvePairLdsMaxN && n > small_max<T>() &&
+         la != nullptr && la->fwd != nullptr && la->fwd_ev != nullptr;
+}
+
+// At n = 32768 with 16 right-hand sides the forward triangle after the factor
+// is ~3.5 ms (512 flag hops); fused, each panel's 16-hop triangle and product
+// run beside the trailing SYRK and only the last one trails the factor.
+template <typename T>
+hipError_t factor_solve_enqueue(T* a, int64_t lda, int n, int m, T* b, int64_t ldb, T* dinv, T* y, SolveSync* sync,
+                                int32_t* info, hipStream_t s, Profiler* prof, const Lookahead* la) {
+  if (!forward_fusable<T>(n, m, la)) {
+    LLH_TRY(potrf_enqueue<T>(a, lda, n, info, false, dinv, s, prof, false, la));
+    if (m <= 0) return hipSuccess;
+    return solve_pair_enqueue<T>(n, a, lda, m, b, ldb, dinv, sync, s, prof, info);
+  }
+  if (y == nullptr) return hipErrorInvalidValue;
+  FwdFuse<T> ff;
+  ff.b = b;
+  ff.ldb = ldb;
+  ff.y = y;
+  ff.ldy = m;
+  ff.m = m;
+  ff.sync = sync;
+  LLH_TRY(potrf_enqueue<T>(a, lda, n, info, false, dinv, s, prof, false, la, nullptr, &ff));
+  LLH_TRY(launch_copy_rows_gated<T>(n, m, y, m, b, ldb, info, s));
+  return trsm_enqueue<T>(true, true, n, a, lda, m, b, ldb, dinv, true, nullptr, sync, s, prof, info);
+}
+
 namespace {
 
 // Half of m rounded to whole kNB blocks (at least one block).
@@ -332,9 +395,15 @@ template hipError_t potri_enqueue<float>(int, const float*, int64_t, const float
                                          float*, hipStream_t);
 
 template hipError_t potrf_enqueue<double>(double*, int64_t, int, int32_t*, bool, double*, hipStream_t,
-                                          Profiler*, bool, const Lookahead*, _Float16*);
+                                          Profiler*, bool, const Lookahead*, _Float16*, const FwdFuse<double>*);
 template hipError_t potrf_enqueue<float>(float*, int64_t, int, int32_t*, bool, float*, hipStream_t, Profiler*,
-                                         bool, const Lookahead*, _Float16*);
+                                         bool, const Lookahead*, _Float16*, const FwdFuse<float>*);
+template bool forward_fusable<double>(int, int, const Lookahead*);
+template bool forward_fusable<float>(int, int, const Lookahead*);
+template hipError_t factor_solve_enqueue<double>(double*, int64_t, int, int, double*, int64_t, double*, double*,
+                                                 SolveSync*, int32_t*, hipStream_t, Profiler*, const Lookahead*);
+template hipError_t factor_solve_enqueue<float>(float*, int64_t, int, int, float*, int64_t, float*, float*,
+                                                SolveSync*, int32_t*, hipStream_t, Profiler*, const Lookahead*);
 template hipError_t solve_pair_enqueue<double>(int, const double*, int64_t, int, double*, int64_t, const double*,
                                                SolveSync*, hipStream_t, Profiler*, const int32_t*);
 template hipError_t solve_pair_enqueue<float>(int, const float*, int64_t, int, float*, int64_t, const float*,

@@ -69,6 +69,18 @@ __global__ void copy_lower_kernel(int m, const T* src, int64_t lds, T* dst, int6
   dst[(int64_t)r * ldd + c] = c <= r ? src[(int64_t)r * lds + c] : T(0);
 }
 
+// dst (rows x m, ldd) = src (lds), unless the gate holds a failed row.  One
+// thread per element; 2-D grid: x over columns in 64-wide strips, y over rows.
+template <typename T>
+__global__ void copy_rows_gated_kernel(int rows, int m, const T* __restrict__ src, int64_t lds, T* __restrict__ dst,
+                                       int64_t ldd, const int32_t* __restrict__ gate) {
+  if (gate != nullptr && *gate != kNoFail) return;
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int r = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (c >= m || r >= rows) return;
+  dst[(int64_t)r * ldd + c] = src[(int64_t)r * lds + c];
+}
+
 template <typename T>
 __global__ void set_identity_kernel(T* a, int64_t lda, int n) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -124,6 +136,20 @@ hipError_t launch_copy_lower(int m, const T* src, int64_t lds, T* dst, int64_t l
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_copy_rows_gated(int rows, int m, const T* src, int64_t lds, T* dst, int64_t ldd,
+                                  const int32_t* gate, hipStream_t s) {
+  if (rows <= 0 || m <= 0) return hipSuccess;
+  if ((rows + 3) / 4 > 65535) return hipErrorInvalidValue;  // grid.y bound
+  hipLaunchKernelGGL((copy_rows_gated_kernel<T>), dim3((m + 63) / 64, (rows + 3) / 4), dim3(256), 0, s, rows, m,
+                     src, lds, dst, ldd, gate);
+  return hipGetLastError();
+}
+
+template hipError_t launch_copy_rows_gated<double>(int, int, const double*, int64_t, double*, int64_t,
+                                                   const int32_t*, hipStream_t);
+template hipError_t launch_copy_rows_gated<float>(int, int, const float*, int64_t, float*, int64_t, const int32_t*,
+                                                  hipStream_t);
 template hipError_t launch_mirror_lower<double>(double*, int64_t, int, hipStream_t);
 template hipError_t launch_mirror_lower<float>(float*, int64_t, int, hipStream_t);
 template hipError_t launch_copy_lower<double>(int, const double*, int64_t, double*, int64_t, hipStream_t);

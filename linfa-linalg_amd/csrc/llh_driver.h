@@ -33,6 +33,27 @@ inline int64_t dinv_elems(int n) { return (int64_t)((n + kNB - 1) / kNB) * kNB *
 struct Lookahead {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  // third stream and its event: the forward substitution fused into the factor
+  hipStream_t fwd = nullptr;
+  hipEvent_t fwd_ev = nullptr;
+};
+
+// Forward substitution fused into the blocked factor (potrf_enqueue with la):
+// y (n x m, ld ldy) first receives a copy of b (ld ldb); then, as each outer
+// panel [k0, k0 + kb) is factored, rows [k0, k0 + kb) of y are solved with the
+// panel's diagonal triangle (one-launch triangle over kb rows, with the
+// diagonal-block inverses) and the rows below take -L21 Y_k (one GEMM).  This
+// runs on la->fwd, ordered after each panel's factor by events, beside the
+// trailing updates; when the factor is complete on s, y = L^-1 B, meaningful
+// only while the status word holds no failure.  Needs m <= 64 and sync.
+template <typename T>
+struct FwdFuse {
+  const T* b = nullptr;
+  int64_t ldb = 0;
+  T* y = nullptr;
+  int64_t ldy = 0;
+  int m = 0;
+  SolveSync* sync = nullptr;
 };
 
 // Enqueues the in-place lower Cholesky of a (row-major, lda) on stream s.
@@ -47,10 +68,12 @@ struct Lookahead {
 // half (f32 storage only, optional): scratch of n * outer_width(n, true) f16 values; each
 // trailing SYRK then rounds its panel to f16 there and multiplies on the f16
 // MFMA with f32 accumulation (the half-precision mixed factor).
+// ff (optional, with la and dinv, blocked sizes only): the fused forward
+// substitution described at FwdFuse.
 template <typename T>
 hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_upper, T* dinv, hipStream_t s,
                          Profiler* prof = nullptr, bool diag_f64 = false, const Lookahead* la = nullptr,
-                         _Float16* half = nullptr);
+                         _Float16* half = nullptr, const FwdFuse<T>* ff = nullptr);
 
 // Enqueues op(A) X = B in place (B row-major n x m); upper_op names the triangle
 // of op(A) and trans != 0 means op(A) = A^T.  dinv, when non-null, holds the
@@ -72,6 +95,23 @@ template <typename T>
 hipError_t solve_pair_enqueue(int n, const T* a, int64_t lda, int m, T* b, int64_t ldb, const T* dinv,
                               SolveSync* sync, hipStream_t s, Profiler* prof = nullptr,
                               const int32_t* gate = nullptr);
+
+// True when factor_solve_enqueue fuses the forward substitution into the
+// factor: two outer panels or more, above the single-workgroup solve's bound,
+// 1 to 64 right-hand sides, and the lookahead streams with their third stream.
+template <typename T>
+bool forward_fusable(int n, int m, const Lookahead* la);
+
+// The blocked factor of a (n > the exact kernel's bound, dinv required) and,
+// for m > 0, x := A^-1 b (B row-major n x m, ldb, in place), on stream s, every
+// solve gated on the status word info (after a failed factor B keeps its
+// values).  When forward_fusable, the forward substitution runs beside the
+// factor into y (n * m elements, FwdFuse), y is copied into B (gated) and only
+// the back substitution follows the factor; otherwise y is unused and the two
+// substitutions follow it.
+template <typename T>
+hipError_t factor_solve_enqueue(T* a, int64_t lda, int n, int m, T* b, int64_t ldb, T* dinv, T* y, SolveSync* sync,
+                                int32_t* info, hipStream_t s, Profiler* prof, const Lookahead* la);
 
 // Enqueues A^-1 = L^-T L^-1 from the factor L (lower triangle of a) and the
 // inverses of its 64 x 64 diagonal blocks (dinv, as potrf_enqueue leaves them):

@@ -264,5 +264,9 @@ hipError_t launch_mirror_lower(T* a, int64_t lda, int n, hipStream_t s);
 // dst (m x m) := the lower triangle of src with zeros above it.
 template <typename T>
 hipError_t launch_copy_lower(int m, const T* src, int64_t lds, T* dst, int64_t ldd, hipStream_t s);
+// dst (rows x m, ldd) := src (lds), skipped whole once gate holds a failed row.
+template <typename T>
+hipError_t launch_copy_rows_gated(int rows, int m, const T* src, int64_t lds, T* dst, int64_t ldd,
+                                  const int32_t* gate, hipStream_t s);
 
 }  // namespace llh

@@ -1,7 +1,8 @@
 // Allocation-free device solvec (llh_dev_solvec_noalloc_d): the same factor
 // and solves as llh_dev_solvec_d, with the scratch those take from the
-// context's grow-only buffers (the inverses of the 64 x 64 diagonal blocks, and
-// the ticket, error word and flags of the one-launch solve) placed in a static
+// context's grow-only buffers (the inverses of the 64 x 64 diagonal blocks, the
+// right-hand sides of the forward substitution fused into the factor, and the
+// ticket, error word and flags of the one-launch solve) placed in a static
 // device array instead.  Each GPU the library runs on holds its own copy of the
 // array until the process ends; llh_context_create locates it once per device
 // (hipGetSymbolAddress with that device current) and every later use goes
@@ -18,9 +19,12 @@ namespace {
 
 constexpr size_t kSyncBytes = 16u * 1024u;          // [0] ticket, [1] error word, [2..] flags
 constexpr size_t kInvBytes = 8u * 1024u * 1024u;    // inverses of the diagonal blocks
-constexpr size_t kScratchBytes = kSyncBytes + kInvBytes;
+constexpr size_t kFwdBytes = 8u * 1024u * 1024u;    // fused forward substitution: n x m, m <= 64
+constexpr size_t kScratchBytes = kSyncBytes + kInvBytes + kFwdBytes;
 static_assert(kScratchBytes == LLH_NOALLOC_SCRATCH_BYTES, "the header states the array's size");
 static_assert(kInvBytes / (llh::kNB * sizeof(double)) == LLH_NOALLOC_MAX_N, "the header states the bound");
+static_assert(kFwdBytes == (size_t)LLH_NOALLOC_MAX_N * 64 * sizeof(double),
+              "the fused forward substitution's right-hand sides fit (it takes m <= 64)");
 // flags of the largest solve: one per (64-row block, 16-column chunk) at m <= 64
 static_assert(2 + (LLH_NOALLOC_MAX_N / llh::kNB) * (64 / llh::kSolveChunkHost) <= kSyncBytes / 4,
               "the flags of the largest one-launch solve fit");
@@ -90,6 +94,7 @@ extern "C" llh_status llh_dev_solvec_noalloc_d(llh_context* ctx, size_t n, doubl
   }
   hipStream_t s = ctx->stream;
   double* dinv = reinterpret_cast<double*>(base + kSyncBytes);
+  double* fwd_y = reinterpret_cast<double*>(base + kSyncBytes + kInvBytes);
   SolveSync sync;
   sync.words = reinterpret_cast<int32_t*>(base);
   sync.nflags = (int64_t)(kSyncBytes / 4) - 2;
@@ -103,14 +108,12 @@ extern "C" llh_status llh_dev_solvec_noalloc_d(llh_context* ctx, size_t n, doubl
     if (e_ != hipSuccess) return hip_fail(err, e_, where);  \
   } while (0)
   NA_TRY(solve_sync_begin(sync, s), "solve sync");
-  NA_TRY(potrf_enqueue<double>(a, (int64_t)lda, (int)n, ctx->d_info, false, dinv, s, nullptr, false, la), "potrf");
+  // the same sequence as llh_dev_solvec_d (solves gated on the factor's status
+  // word); the fused forward substitution, where it applies, takes m <= 64
+  NA_TRY(factor_solve_enqueue<double>(a, (int64_t)lda, (int)n, (int)m, b, (int64_t)ldb, dinv, fwd_y, &sync,
+                                      ctx->d_info, s, nullptr, la),
+         "factor and solve");
   NA_TRY(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, s), "status");
-  if (m > 0) {
-    // gated on the factor's status word, as llh_dev_solvec_d
-    NA_TRY(solve_pair_enqueue<double>((int)n, a, (int64_t)lda, (int)m, b, (int64_t)ldb, dinv, &sync, s, nullptr,
-                                      ctx->d_info),
-           "solve");
-  }
   NA_TRY(hipMemcpyAsync(ctx->h_info + 2, sync.words + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s),
          "solve status");
   NA_TRY(hipStreamSynchronize(s), "synchronize");

@@ -476,3 +476,37 @@ def test_dev_solvec_solve_pair_forms(n, m, dtype, tol):
     with pytest.raises(llh.NotPositiveDefinite):
         dev.solvec(torch.from_numpy(bad).cuda(), tb2)
     assert np.array_equal(tb2.cpu().numpy(), b)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 5e-4)])
+@pytest.mark.parametrize("n", [1025, 1600])
+@pytest.mark.parametrize("m", [1, 16, 17, 64])
+def test_dev_solvec_fused_forward(n, m, dtype, tol):
+    """From two outer panels of 512 columns on, with at most 64 right-hand
+    sides, solvec runs the forward substitution beside the factor, one panel at
+    a time (a triangle over the panel's rows, then one product for the rows
+    below), into a workspace: x still matches the oracle's (ragged last panel,
+    1 / 16 / 17 / 64 columns, B strided), B's padding is kept, and after a
+    failed factor, early or in the last panel, B is untouched (cholesky.rs:140);
+    the host-memory form takes the same path."""
+    import torch
+    from linfa_linalg_hip import device as dev
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    a = spd(n, 7 * n + m, dtype)
+    b = np.random.default_rng(3 * n + m).standard_normal((n, m)).astype(dtype)
+    x_ref = orc.solvec(np.array(a, dtype=np.float64), np.array(b, dtype=np.float64))
+    tb_wide = torch.full((n, m + 5), 7.0, dtype=tdt, device="cuda")
+    tb_wide[:, :m] = torch.from_numpy(b).cuda()
+    dev.solvec(torch.from_numpy(a).cuda(), tb_wide[:, :m])
+    x = tb_wide[:, :m].cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= tol, (n, m)
+    assert torch.all(tb_wide[:, m:] == 7.0)
+    xp = np.asarray(llh.solvec(np.array(a), np.array(b)), dtype=np.float64)
+    assert np.linalg.norm(xp - x_ref) / np.linalg.norm(x_ref) <= tol, (n, m)
+    for row in (100, n - 3):
+        bad = np.array(a)
+        bad[row, row] = -1.0
+        tb2 = torch.from_numpy(b).cuda()
+        with pytest.raises(llh.NotPositiveDefinite):
+            dev.solvec(torch.from_numpy(bad).cuda(), tb2)
+        assert np.array_equal(tb2.cpu().numpy(), b), row
