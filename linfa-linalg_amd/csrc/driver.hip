@@ -136,8 +136,9 @@ hipError_t potrf_enqueue(T* a, int64_t lda, int n, int32_t* info, bool zero_uppe
     if (ff != nullptr) {
       LLH_TRY(hipEventRecord(la->fwd_ev, s));
       LLH_TRY(hipStreamWaitEvent(la->fwd, la->fwd_ev, 0));
-      LLH_TRY(hipMemcpy2DAsync(ff->y, ff->ldy * sizeof(T), ff->b, ff->ldb * sizeof(T), ff->m * sizeof(T), n,
-                               hipMemcpyDeviceToDevice, la->fwd));
+      if (ff->b != nullptr)
+        LLH_TRY(hipMemcpy2DAsync(ff->y, ff->ldy * sizeof(T), ff->b, ff->ldb * sizeof(T), ff->m * sizeof(T), n,
+                                 hipMemcpyDeviceToDevice, la->fwd));
       LLH_TRY(fwd_panel(0, std::min(nbo, n)));
     }
     for (int k0 = 0; k0 < n; k0 += nbo) {

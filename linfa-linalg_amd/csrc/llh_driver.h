@@ -39,7 +39,8 @@ struct Lookahead {
 };
 
 // Forward substitution fused into the blocked factor (potrf_enqueue with la):
-// y (n x m, ld ldy) first receives a copy of b (ld ldb); then, as each outer
+// y (n x m, ld ldy) first receives a copy of b (ld ldb; with b null, y already
+// holds the right-hand sides, written in stream order on s); then, as each outer
 // panel [k0, k0 + kb) is factored, rows [k0, k0 + kb) of y are solved with the
 // panel's diagonal triangle (one-launch triangle over kb rows, with the
 // diagonal-block inverses) and the rows below take -L21 Y_k (one GEMM).  This

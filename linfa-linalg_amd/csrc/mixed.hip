@@ -70,12 +70,25 @@ hipError_t grow_buffer(llh_context* ctx, bool owns, llh_context::Buffer* borrowe
   return hipSuccess;
 }
 
+// The first right-hand sides of a one-shot solve, for the forward substitution
+// fused into the low-precision factor: b (n x m f64, ldb) is rounded (and, for
+// the f16 factor, scaled) into y (n x m f32, ld m), and y = L^-1 y is formed
+// beside the factor.  done tells whether the factor took it (a fallback to f64
+// or a size the fusion does not cover leaves y unused).
+struct FirstRhs {
+  int m = 0;
+  const double* b = nullptr;
+  size_t ldb = 0;
+  float* y = nullptr;
+  bool done = false;
+};
+
 // Factors f->a into f->l with precision f->prec; on a low-precision pivot
 // failure with fallback, refactors in f64.  The f16 form factors
 // diag(s) A diag(s) with s = diag(A)^-1/2 (unit diagonal, every entry of L at
 // most 1 in magnitude, so nothing overflows f16), keeping the factor in f32
 // and rounding each trailing SYRK's panel to f16.
-llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error* err) {
+llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error* err, FirstRhs* first = nullptr) {
   const int n = f->n;
   hipStream_t s = ctx->stream;
   // work: [rowsum n | norm 2] [partial sums of the split row-sum walk]
@@ -111,9 +124,26 @@ llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error
         half = static_cast<_Float16*>(hp);
       }
       CHECK_HIP(launch_convert_d2s(n, n, f->a, f->lda, l, n, true, s, sc, sc), "convert");
+      const Lookahead* la = ctx_lookahead(ctx);
+      FwdFuse<float> ff;
+      SolveSync sync;
+      const bool fuse = first != nullptr && first->y != nullptr && forward_fusable<float>(n, first->m, la);
+      if (fuse) {
+        CHECK_HIP(ctx_sync(ctx, n, first->m, &sync), "solve sync");
+        CHECK_HIP(launch_convert_d2s(n, first->m, first->b, (int64_t)first->ldb, first->y, first->m, false, s, sc),
+                  "convert rhs");
+        ff.y = first->y;
+        ff.ldy = first->m;
+        ff.m = first->m;
+        ff.sync = &sync;
+      }
       CHECK_HIP(potrf_enqueue<float>(l, n, n, ctx->d_info, false, static_cast<float*>(f->dinv), s,
-                                     ctx->profiler(), true, ctx_lookahead(ctx), half),
+                                     ctx->profiler(), true, la, half, fuse ? &ff : nullptr),
                 "potrf f32");
+      if (fuse) {
+        CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
+        first->done = true;
+      }
     } else {
       double* l = static_cast<double*>(f->l);
       CHECK_HIP(hipMemcpy2DAsync(l, (size_t)n * 8, f->a, (size_t)f->lda * 8, (size_t)n * 8, n,
@@ -126,6 +156,8 @@ llh_status factor_core(llh_context* ctx, llh_factor* f, int32_t flags, llh_error
     CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, s), "status");
     CHECK_HIP(hipStreamSynchronize(s), "synchronize");
     const int32_t fail = ctx->h_info[0];
+    if (first != nullptr && first->done && (fail != kNoFail || ctx->h_info[2] != 0))
+      first->done = false;  // y is not L^-1 b: the solve starts over
     if (fail == kNoFail) return ok(err);
     if (low_precision(f->prec) && (flags & LLH_FACTOR_FALLBACK_F64)) {
       f->prec = LLH_PREC_F64;
@@ -148,8 +180,11 @@ hipError_t solve_low(llh_context* ctx, const llh_factor* f, int m, T* d, SolveSy
   return solve_pair_enqueue<T>(n, l, n, m, d, m, dinv, sync, ctx->stream, ctx->profiler());
 }
 
+// y_fwd (optional, low precision only): L^-1 of b's rounding, left by the
+// factor (FirstRhs); the first solve then takes only the back substitution.
 llh_status solve_core(llh_context* ctx, llh_factor* f, size_t m_, const double* b, size_t ldb, double* x,
-                      size_t ldx, const llh_refine_opts* opts, llh_refine_info* info, llh_error* err) {
+                      size_t ldx, const llh_refine_opts* opts, llh_refine_info* info, llh_error* err,
+                      float* y_fwd = nullptr) {
   const int n = f->n, m = (int)m_;
   hipStream_t s = ctx->stream;
   llh_refine_info inf{0, 0, 0, (int32_t)f->prec, 0.0, f->anrm};
@@ -210,10 +245,16 @@ llh_status solve_core(llh_context* ctx, llh_factor* f, size_t m_, const double* 
     // The f16 factor is of S A S (S = diag(scale)): A^-1 v = S (S A S)^-1 S v.
     const double* sc = f->prec == LLH_PREC_F16 ? static_cast<const double*>(f->scale) : nullptr;
     // x = solve_low(b)
-    CHECK_HIP(launch_convert_d2s(n, m, b, (int64_t)ldb, d, m, false, s, sc), "convert rhs");
-    CHECK_HIP(solve_low<float>(ctx, f, m, d, &sync), "solve f32");
+    if (y_fwd != nullptr) {
+      CHECK_HIP(trsm_enqueue<float>(true, true, n, static_cast<float*>(f->l), n, m, y_fwd, m,
+                                    static_cast<float*>(f->dinv), true, nullptr, &sync, s, ctx->profiler()),
+                "solve f32");
+    } else {
+      CHECK_HIP(launch_convert_d2s(n, m, b, (int64_t)ldb, d, m, false, s, sc), "convert rhs");
+      CHECK_HIP(solve_low<float>(ctx, f, m, d, &sync), "solve f32");
+    }
     CHECK_HIP(launch_fill_i32(reinterpret_cast<int32_t*>(x), 0, 2 * (int64_t)n * m, s), "zero x");
-    CHECK_HIP(launch_axpy_s2d(n, m, d, m, x, (int64_t)ldx, s, sc), "update x");
+    CHECK_HIP(launch_axpy_s2d(n, m, y_fwd != nullptr ? y_fwd : d, m, x, (int64_t)ldx, s, sc), "update x");
     double prev = INFINITY;
     for (int it = 0;; ++it) {
       bool conv = false;
@@ -276,13 +317,19 @@ llh_status llh_factor_create(llh_context* ctx, llh_precision prec, size_t n, con
   return LLH_OK;
 }
 
-llh_status llh_factor_solve(llh_context* ctx, llh_factor* f, size_t m, const double* b, size_t ldb, double* x,
-                            size_t ldx, const llh_refine_opts* opts, llh_refine_info* info, llh_error* err) {
+}  // extern "C"
+
+namespace {
+
+// llh_factor_solve, its first solve optionally started by the factor (y_fwd).
+llh_status factor_solve_impl(llh_context* ctx, llh_factor* f, size_t m, const double* b, size_t ldb, double* x,
+                             size_t ldx, const llh_refine_opts* opts, llh_refine_info* info, llh_error* err,
+                             float* y_fwd) {
   if (bad_ctx(ctx)) return invalid(err, "null or destroyed context");
   if (f == nullptr || f->magic != llh_factor::kMagic) return invalid(err, "null or destroyed factor");
   if (m == 0) return ok(err);
   if (b == nullptr || x == nullptr || ldb < m || ldx < m) return invalid(err, "bad right-hand side arguments");
-  llh_status st = solve_core(ctx, f, m, b, ldb, x, ldx, opts, info, err);
+  llh_status st = solve_core(ctx, f, m, b, ldb, x, ldx, opts, info, err, y_fwd);
   if (st == LLH_ERR_NO_CONVERGENCE && opts && (opts->flags & LLH_FACTOR_FALLBACK_F64) &&
       low_precision(f->prec)) {
     f->prec = LLH_PREC_F64;
@@ -292,6 +339,15 @@ llh_status llh_factor_solve(llh_context* ctx, llh_factor* f, size_t m, const dou
     if (info) info->fell_back = 1;
   }
   return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+llh_status llh_factor_solve(llh_context* ctx, llh_factor* f, size_t m, const double* b, size_t ldb, double* x,
+                            size_t ldx, const llh_refine_opts* opts, llh_refine_info* info, llh_error* err) {
+  return factor_solve_impl(ctx, f, m, b, ldb, x, ldx, opts, info, err, nullptr);
 }
 
 llh_precision llh_factor_precision(const llh_factor* f) {
@@ -328,11 +384,26 @@ llh_status llh_dev_solve_mixed(llh_context* ctx, llh_precision prec, size_t n, c
   f.lda = (int64_t)lda;
   f.owns = false;
   const int32_t flags = opts ? opts->flags : 0;
-  st = factor_core(ctx, &f, flags, err);
+  // The first forward substitution runs beside the low-precision factor where
+  // the fusion covers the size (one-launch triangles, m <= 64).
+  FirstRhs first;
+  if (low_precision(prec) && m > 0 && m <= 64 && b != nullptr && x != nullptr && ldb >= m && ldx == m &&
+      forward_fusable<float>((int)n, (int)m, ctx_lookahead(ctx))) {
+    void* yp = nullptr;
+    size_t yb = 0;
+    if (grow_buffer(ctx, false, &ctx->fwd_y, &yp, &yb, n * m * sizeof(float)) == hipSuccess) {
+      first.m = (int)m;
+      first.b = b;
+      first.ldb = ldb;
+      first.y = static_cast<float*>(yp);
+    }
+  }
+  st = factor_core(ctx, &f, flags, err, &first);
   const bool fell = low_precision(prec) && f.prec == LLH_PREC_F64;
   if (st != LLH_OK) return st;
   if (m > 0) {
-    st = llh_factor_solve(ctx, &f, m, b, ldb, x, ldx, opts, info, err);
+    st = factor_solve_impl(ctx, &f, m, b, ldb, x, ldx, opts, info, err,
+                           first.done && low_precision(f.prec) ? first.y : nullptr);
     if (info && fell) info->fell_back = 1;
   }
   f.magic = 0;  // borrowed storage stays with the context

@@ -125,6 +125,33 @@ def test_fallback_to_f64_on_f32_breakdown():
         dev.solve_mixed(tt(a), tt(b), x2, prec="f32", fallback=False, max_iter=5)
 
 
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_fallback_at_fused_size(prec):
+    """From two outer panels of 512 columns on, the first forward substitution
+    runs beside the low-precision factor; when that factor breaks down (or its
+    refinement stalls) the f64 fallback solves from scratch and still reaches
+    f64 accuracy, and the well-conditioned case at the same size matches the
+    oracle."""
+    from linfa_linalg_hip import device as dev
+    import torch
+    n = 1100
+    rng = np.random.default_rng(17)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    a = (q * np.logspace(0, -11, n)) @ q.T
+    a = (a + a.T) / 2
+    b = rng.standard_normal((n, 3))
+    x = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    info = dev.solve_mixed(tt(a), tt(b), x, prec=prec, fallback=True)
+    assert info["fell_back"] and info["precision_used"] == "f64", info
+    assert info["residual"] <= 10 * np.sqrt(n) * np.finfo(np.float64).eps
+    good = spd_normal(n, 1.0, 23)
+    x2 = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    info2 = dev.solve_mixed(tt(good), tt(b), x2, prec=prec, fallback=False)
+    assert info2["converged"] and not info2["fell_back"], info2
+    x_ref = orc.solvec(np.array(good), b)
+    assert np.linalg.norm(x2.cpu().numpy() - x_ref) / np.linalg.norm(x_ref) <= 1e-12
+
+
 def test_not_positive_definite():
     from linfa_linalg_hip import device as dev
     import torch
