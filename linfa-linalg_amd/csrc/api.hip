@@ -197,37 +197,74 @@ void scatter(const T* in, size_t rows, size_t cols, T* a, ptrdiff_t rs, ptrdiff_
 
 bool bad_ctx(llh_context* ctx) { return ctx == nullptr || ctx->magic != llh_context::kMagic; }
 
+// Host staging buffer of the plain forms for count elements: a page-locked,
+// grow-only context buffer (slot 0 or 1) up to kPinnedStageMax bytes, since
+// copies from page-locked memory go straight over the link where pageable ones
+// are bounced through the runtime's own buffers (n = 512: the plain solvec's
+// two 2 MB copies); the heap vector beyond that.
+constexpr size_t kPinnedStageMax = 64u << 20;
+template <typename T>
+hipError_t ctx_stage(llh_context* ctx, int slot, size_t count, std::vector<T>& heap, T** out) {
+  const size_t bytes = count * sizeof(T);
+  llh_context::Buffer& b = ctx->host_stage[slot];
+  if (bytes > kPinnedStageMax) {
+    heap.resize(count);
+    *out = heap.data();
+    return hipSuccess;
+  }
+  if (b.bytes < bytes) {
+    if (b.ptr != nullptr) {
+      hipError_t e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) return e;
+      (void)hipHostFree(b.ptr);
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    const size_t want = std::max<size_t>(bytes, 1u << 16);
+    hipError_t e = hipHostMalloc(&b.ptr, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      heap.resize(count);
+      *out = heap.data();
+      return hipSuccess;
+    }
+    b.bytes = want;
+  }
+  *out = static_cast<T*>(b.ptr);
+  return hipSuccess;
+}
+
 // Factors the packed host copy `h` (n x n row-major) on the device and writes
 // the result back into the user's view with the reference's semantics.
 // Returns the first failing row or -1.
 template <typename T>
 llh_status factor_plain(llh_context* ctx, size_t n, T* a, ptrdiff_t rs, ptrdiff_t cs, bool clean,
-                        std::vector<T>& h, T** d_out, int64_t* fail_out, llh_error* err) {
+                        T* h, T** d_out, int64_t* fail_out, llh_error* err) {
   void* dv = nullptr;
   CHECK_HIP(ctx_buffer(ctx, 0, n * n * sizeof(T), &dv), "device buffer");
   T* d = static_cast<T*>(dv);
   void* dinv = nullptr;
   CHECK_HIP(ctx_dinv(ctx, (int)n, sizeof(T), &dinv), "inverse buffer");
-  CHECK_HIP(hipMemcpyAsync(d, h.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  CHECK_HIP(hipMemcpyAsync(d, h, n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
   CHECK_HIP(potrf_enqueue<T>(d, (int64_t)n, (int)n, ctx->d_info, clean, static_cast<T*>(dinv), ctx->stream, nullptr,
                             false, ctx_lookahead(ctx)),
             "potrf");
-  CHECK_HIP(hipMemcpyAsync(h.data(), d, n * n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
+  CHECK_HIP(hipMemcpyAsync(h, d, n * n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
             "status");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
   const int32_t f = ctx->h_info[0];
   if (f == kNoFail) {
     if (clean)
-      scatter(h.data(), n, n, a, rs, cs, [n](size_t) { return n; });
+      scatter(h, n, n, a, rs, cs, [n](size_t) { return n; });
     else
-      scatter(h.data(), n, n, a, rs, cs, [](size_t i) { return i + 1; });
+      scatter(h, n, n, a, rs, cs, [](size_t i) { return i + 1; });
     *fail_out = -1;
   } else {
     // cholesky.rs:54-72 returns at row f: rows < f are factored, row f holds
     // L[f, ..f], the rest of the array is as it was.
     const size_t fr = (size_t)f;
-    scatter(h.data(), fr + 1, n, a, rs, cs, [fr](size_t i) { return i < fr ? i + 1 : fr; });
+    scatter(h, fr + 1, n, a, rs, cs, [fr](size_t i) { return i < fr ? i + 1 : fr; });
     *fail_out = f;
   }
   if (d_out) *d_out = d;
@@ -243,8 +280,10 @@ llh_status cholesky_plain(llh_context* ctx, size_t rows, size_t cols, T* a, ptrd
   if (n == 0) return ok(err);
   if (a == nullptr) return invalid(err, "null matrix pointer");
   if (n > 0x7fffffff) return invalid(err, "matrix too large");
-  std::vector<T> h(n * n);
-  gather(a, rs, cs, n, n, h.data());
+  std::vector<T> hv;
+  T* h = nullptr;
+  CHECK_HIP(ctx_stage<T>(ctx, 0, n * n, hv, &h), "host staging");
+  gather(a, rs, cs, n, n, h);
   int64_t fail = -1;
   llh_status st = factor_plain<T>(ctx, n, a, rs, cs, clean, h, nullptr, &fail, err);
   if (st != LLH_OK) return st;
@@ -316,9 +355,12 @@ llh_status solve_triangular_plain(llh_context* ctx, size_t rows, size_t cols, co
   if (b_rows != n) return wrong_rows(err, n, b_rows);  // triangular.rs:103-108
   if (n == 0 || b_cols == 0) return ok(err);
   if (a == nullptr || b == nullptr) return invalid(err, "null matrix pointer");
-  std::vector<T> ha(n * n), hb(n * b_cols);
-  gather(a, rs, cs, n, n, ha.data());
-  gather(b, brs, bcs, n, b_cols, hb.data());
+  std::vector<T> hav, hbv;
+  T *ha = nullptr, *hb = nullptr;
+  CHECK_HIP(ctx_stage<T>(ctx, 0, n * n, hav, &ha), "host staging");
+  CHECK_HIP(ctx_stage<T>(ctx, 1, n * b_cols, hbv, &hb), "host staging");
+  gather(a, rs, cs, n, n, ha);
+  gather(b, brs, bcs, n, b_cols, hb);
   if (diag != nullptr) {
     // diag_fn: the kernel (triangular.rs:113-124) reads a's diagonal only as the
     // pivot, so the packed copy carries the caller's pivots in its place
@@ -332,8 +374,8 @@ llh_status solve_triangular_plain(llh_context* ctx, size_t rows, size_t cols, co
   CHECK_HIP(ctx_buffer(ctx, 1, n * b_cols * sizeof(T), &dbv), "device buffer");
   T* da = static_cast<T*>(dav);
   T* db = static_cast<T*>(dbv);
-  CHECK_HIP(hipMemcpyAsync(da, ha.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
-  CHECK_HIP(hipMemcpyAsync(db, hb.data(), n * b_cols * sizeof(T), hipMemcpyHostToDevice, ctx->stream),
+  CHECK_HIP(hipMemcpyAsync(da, ha, n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  CHECK_HIP(hipMemcpyAsync(db, hb, n * b_cols * sizeof(T), hipMemcpyHostToDevice, ctx->stream),
             "upload");
   void* dinv = nullptr;
   CHECK_HIP(ctx_dinv(ctx, (int)n, sizeof(T), &dinv), "inverse buffer");
@@ -344,11 +386,11 @@ llh_status solve_triangular_plain(llh_context* ctx, size_t rows, size_t cols, co
                             ctx->profiler()),
             "trsm");
   CHECK_HIP(ctx_read_solve_error(ctx), "solve status");
-  CHECK_HIP(hipMemcpyAsync(hb.data(), db, n * b_cols * sizeof(T), hipMemcpyDeviceToHost, ctx->stream),
+  CHECK_HIP(hipMemcpyAsync(hb, db, n * b_cols * sizeof(T), hipMemcpyDeviceToHost, ctx->stream),
             "download");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
   if (ctx->h_info[2] != 0) return solve_stalled(err);
-  scatter(hb.data(), n, b_cols, b, brs, bcs, [b_cols](size_t) { return b_cols; });
+  scatter(hb, n, b_cols, b, brs, bcs, [b_cols](size_t) { return b_cols; });
   return ok(err);
 }
 
@@ -435,8 +477,10 @@ llh_status solvec_plain(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdif
   }
   if (a == nullptr || (!invert && b == nullptr && b_cols > 0)) return invalid(err, "null matrix pointer");
   if (n > 0x7fffffff || m > 0x7fffffff) return invalid(err, "matrix too large");
-  std::vector<T> ha(n * n);
-  gather(a, rs, cs, n, n, ha.data());
+  std::vector<T> hav, hbv;
+  T* ha = nullptr;
+  CHECK_HIP(ctx_stage<T>(ctx, 0, n * n, hav, &ha), "host staging");
+  gather(a, rs, cs, n, n, ha);
   const bool rows_ok = invert || b_rows == n;
   if (!rows_ok || m == 0) {
     // The factor runs (and A keeps the dirty factor) before WrongRows is
@@ -448,13 +492,14 @@ llh_status solvec_plain(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdif
     if (!rows_ok) return wrong_rows(err, n, b_rows);
     return ok(err);
   }
-  std::vector<T> hb(n * m);
+  T* hb = nullptr;
+  CHECK_HIP(ctx_stage<T>(ctx, 1, n * m, hbv, &hb), "host staging");
   void *dav = nullptr, *dbv = nullptr;
   CHECK_HIP(ctx_buffer(ctx, 0, n * n * sizeof(T), &dav), "device buffer");
   CHECK_HIP(ctx_buffer(ctx, 1, n * m * sizeof(T), &dbv), "device buffer");
   T* da = static_cast<T*>(dav);
   T* db = static_cast<T*>(dbv);
-  CHECK_HIP(hipMemcpyAsync(da, ha.data(), n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+  CHECK_HIP(hipMemcpyAsync(da, ha, n * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
   llh_status st = LLH_OK;
   if (invert && (int)n > small_max<T>()) {
     st = invert_on_device<T>(ctx, n, da, n, db, n, err);
@@ -462,29 +507,29 @@ llh_status solvec_plain(llh_context* ctx, size_t rows, size_t cols, T* a, ptrdif
     if (invert) {
       CHECK_HIP(launch_set_identity<T>(db, (int64_t)n, (int)n, ctx->stream), "identity");  // cholesky.rs:179
     } else {
-      gather(b, brs, bcs, n, m, hb.data());
-      CHECK_HIP(hipMemcpyAsync(db, hb.data(), n * m * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
+      gather(b, brs, bcs, n, m, hb);
+      CHECK_HIP(hipMemcpyAsync(db, hb, n * m * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "upload");
     }
     st = solve_on_device<T>(ctx, n, da, m, db, err);
   }
   if (st != LLH_OK) return st;
-  CHECK_HIP(hipMemcpyAsync(ha.data(), da, n * n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
-  CHECK_HIP(hipMemcpyAsync(hb.data(), db, n * m * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
+  CHECK_HIP(hipMemcpyAsync(ha, da, n * n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
+  CHECK_HIP(hipMemcpyAsync(hb, db, n * m * sizeof(T), hipMemcpyDeviceToHost, ctx->stream), "download");
   CHECK_HIP(hipMemcpyAsync(ctx->h_info, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream),
             "status");
   CHECK_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
   const int32_t f = ctx->h_info[0];
   if (f != kNoFail) {
     const size_t fr = (size_t)f;
-    scatter(ha.data(), fr + 1, n, a, rs, cs, [fr](size_t i) { return i < fr ? i + 1 : fr; });
+    scatter(ha, fr + 1, n, a, rs, cs, [fr](size_t i) { return i < fr ? i + 1 : fr; });
     return not_pd(err, f);
   }
   if (ctx->h_info[2] != 0) return solve_stalled(err);
-  scatter(ha.data(), n, n, a, rs, cs, [](size_t i) { return i + 1; });  // dirty factor
+  scatter(ha, n, n, a, rs, cs, [](size_t i) { return i + 1; });  // dirty factor
   if (invert)
-    scatter(hb.data(), n, n, inv_out, ors, ocs, [n](size_t) { return n; });
+    scatter(hb, n, n, inv_out, ors, ocs, [n](size_t) { return n; });
   else
-    scatter(hb.data(), n, m, b, brs, bcs, [m](size_t) { return m; });
+    scatter(hb, n, m, b, brs, bcs, [m](size_t) { return m; });
   return ok(err);
 }
 
@@ -743,6 +788,8 @@ void llh_context_destroy(llh_context* ctx) {
   if (ctx->half_panel.ptr) (void)hipFree(ctx->half_panel.ptr);
   if (ctx->sync.ptr) (void)hipFree(ctx->sync.ptr);
   if (ctx->fwd_y.ptr) (void)hipFree(ctx->fwd_y.ptr);
+  for (auto& b : ctx->host_stage)
+    if (b.ptr) (void)hipHostFree(b.ptr);
   if (ctx->d_info) (void)hipFree(ctx->d_info);
   if (ctx->h_info) (void)hipHostFree(ctx->h_info);
   if (ctx->la.side) {

@@ -29,6 +29,7 @@ struct llh_context {
   Buffer half_panel;  // f16 copy of the panel of the f16 factor's trailing SYRK
   Buffer sync;        // ticket / error / flags of the one-launch solve
   Buffer fwd_y;       // right-hand sides of the forward substitution fused into the factor
+  Buffer host_stage[2];  // page-locked host staging of the plain forms (hipHostMalloc)
   int32_t epoch = 1;
   int64_t spin_limit = llh::kSpinLimit;  // bound on the one-launch solve's flag waits
   llh::Lookahead la;  // side stream and events of the factor's lookahead (created on first use)
