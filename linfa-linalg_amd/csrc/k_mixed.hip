@@ -74,16 +74,20 @@ __global__ void diag_rsqrt_kernel(int n, const double* __restrict__ A, int64_t l
 // weighted row sums of |A|), A symmetric from its lower triangle.
 //
 // Workgroup (i, q, sp): 64 rows of R (row block i), 16 columns (chunk q), and
-// the blocks j of split sp of row block i of the full symmetric A.  Lane r of
-// wave v holds row r's 16 values of block j at k = 16v .. 16v + 15 in
-// registers, loaded two blocks ahead: blocks left of the diagonal row-wise from
-// the lower triangle, blocks right of it as transposed lower blocks (lane r
-// reads column i0 + r, so a wave's load is one contiguous row segment), the
-// diagonal block from both halves.  X_j goes through a double-buffered LDS slab,
-// one barrier per block; the four wave partials are reduced once at the end.
-// With one split the workgroup writes R; with several it writes its partial sum
-// to P[sp] and symm_reduce_kernel adds the splits in order, so the result does
-// not depend on scheduling either way.
+// the blocks j of split sp of row block i of the full symmetric A.  Lane l of
+// wave v holds rows p = l mod 32 and p + 32 of block j at the 8 columns
+// k = 16v + 8h .. 16v + 8h + 7, h = l / 32, in registers, loaded one block
+// ahead: blocks left of the diagonal row-wise from the lower triangle, blocks
+// right of it as transposed lower blocks (for each k the 32 lanes of a half
+// read 32 consecutive entries of one row), the diagonal block from both halves.
+// X_j goes through a double-buffered LDS slab, one barrier per block; every
+// value read from it (one address per half-wave) feeds two FMAs, one per row.
+// With one row per lane and 16 k those broadcast reads bounded the walk (128
+// ds_read_b128 per wave and block; ~4.2 TB/s of A at n = 32768).  The halves'
+// partial sums are added by one lane exchange and the four waves' once through
+// LDS at the end.  With one split the workgroup writes R; with several it
+// writes its partial sum to P[sp] and symm_reduce_kernel adds the splits in
+// order, so the result does not depend on scheduling either way.
 constexpr int kSymmMC = 16;
 
 template <bool ABS, bool VEC>
@@ -92,7 +96,7 @@ __global__ __launch_bounds__(256) void symm_walk_kernel(int n, int m, const doub
                                                         const double* __restrict__ B, int64_t ldb,
                                                         double* __restrict__ R, int64_t ldr,
                                                         double* __restrict__ P, int nsplit, int mp) {
-  constexpr int MC = kSymmMC, KW = kNB / 4, SL = kNB * MC;
+  constexpr int MC = kSymmMC, KH = kNB / 8, SL = kNB * MC;  // KH: k per lane
   __shared__ __attribute__((aligned(16))) double Ys[2][SL];
   __shared__ double Part[3][SL];
   const int i = blockIdx.x, q = blockIdx.y, sp = blockIdx.z;
@@ -101,40 +105,54 @@ __global__ __launch_bounds__(256) void symm_walk_kernel(int n, int m, const doub
   const int jb = sp * per, je = min(nb, jb + per);
   const int i0 = i * kNB, bi = min(kNB, n - i0);
   const int c0 = q * MC, mc = ABS ? 1 : min(MC, m - c0);
-  const int tid = threadIdx.x, r = tid & 63, wv = tid >> 6, k0 = KW * wv;
-  const int rr = min(r, bi - 1);  // clamped row: rows past the matrix compute and are not written
+  const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
+  const int p = l & 31, h = l >> 5, k0 = 16 * wv + KH * h;
+  // clamped rows p and p + 32: rows past the matrix compute and are not written
+  const int rr0 = min(p, bi - 1), rr1 = min(p + 32, bi - 1);
   typedef double v2d __attribute__((ext_vector_type(2)));
+  // out[0 .. KH) row p, out[KH .. 2 KH) row p + 32
   auto fetch = [&](int j, double* out) {
     const int j0 = j * kNB, bj = min(kNB, n - j0);
     if (j < i) {  // A[i0 + r][j0 + k]
-      const double* row = A + (int64_t)(i0 + rr) * lda + j0 + k0;
-      if (VEC && k0 + KW <= bj) {
 #pragma unroll
-        for (int v = 0; v < KW / 2; ++v) {
-          const v2d x = *reinterpret_cast<const v2d*>(row + 2 * v);
-          out[2 * v] = x[0];
-          out[2 * v + 1] = x[1];
+      for (int s = 0; s < 2; ++s) {
+        const double* row = A + (int64_t)(i0 + (s ? rr1 : rr0)) * lda + j0 + k0;
+        if (VEC && k0 + KH <= bj) {
+#pragma unroll
+          for (int v = 0; v < KH / 2; ++v) {
+            const v2d x = *reinterpret_cast<const v2d*>(row + 2 * v);
+            out[s * KH + 2 * v] = x[0];
+            out[s * KH + 2 * v + 1] = x[1];
+          }
+        } else {
+#pragma unroll
+          for (int kk = 0; kk < KH; ++kk) out[s * KH + kk] = (k0 + kk < bj) ? row[kk] : 0.0;
         }
-      } else {
-#pragma unroll
-        for (int kk = 0; kk < KW; ++kk) out[kk] = (k0 + kk < bj) ? row[kk] : 0.0;
       }
     } else if (j > i) {  // A[j0 + k][i0 + r]
-      const double* col = A + (int64_t)(j0 + k0) * lda + i0 + rr;
 #pragma unroll
-      for (int kk = 0; kk < KW; ++kk) out[kk] = (k0 + kk < bj) ? col[(int64_t)kk * lda] : 0.0;
+      for (int s = 0; s < 2; ++s) {
+        const double* col = A + (int64_t)(j0 + k0) * lda + i0 + (s ? rr1 : rr0);
+#pragma unroll
+        for (int kk = 0; kk < KH; ++kk) out[s * KH + kk] = (k0 + kk < bj) ? col[(int64_t)kk * lda] : 0.0;
+      }
     } else {  // diagonal block, from its lower half
 #pragma unroll
-      for (int kk = 0; kk < KW; ++kk) {
-        const int k = k0 + kk;
-        const int64_t o = k <= rr ? (int64_t)(i0 + rr) * lda + i0 + k : (int64_t)(i0 + min(k, bi - 1)) * lda + i0 + rr;
-        const double v = A[o];
-        out[kk] = k < bi ? v : 0.0;
+      for (int s = 0; s < 2; ++s) {
+        const int rr = s ? rr1 : rr0;
+#pragma unroll
+        for (int kk = 0; kk < KH; ++kk) {
+          const int k = k0 + kk;
+          const int64_t o =
+              k <= rr ? (int64_t)(i0 + rr) * lda + i0 + k : (int64_t)(i0 + min(k, bi - 1)) * lda + i0 + rr;
+          const double v = A[o];
+          out[s * KH + kk] = k < bi ? v : 0.0;
+        }
       }
     }
     if (ABS) {
 #pragma unroll
-      for (int kk = 0; kk < KW; ++kk) out[kk] = fabs(out[kk]);
+      for (int kk = 0; kk < 2 * KH; ++kk) out[kk] = fabs(out[kk]);
     }
   };
   // X_j rows [0, 64) x chunk columns: 4 values per thread (row e / 16, column e % 16)
@@ -150,56 +168,66 @@ __global__ __launch_bounds__(256) void symm_walk_kernel(int n, int m, const doub
     }
   };
 
-  double acc[MC];
+  double acc0[MC], acc1[MC];
 #pragma unroll
-  for (int c = 0; c < MC; ++c) acc[c] = 0.0;
-  double a0[KW], a1[KW], a2[KW], y0[4], y1[4];
+  for (int c = 0; c < MC; ++c) acc0[c] = acc1[c] = 0.0;
+  double a0[2 * KH], a1[2 * KH], y0[4];
   if (jb < je) {
     fetch(jb, a0);
     loady(jb, y0);
-  }
-  if (jb + 1 < je) {
-    fetch(jb + 1, a1);
-    loady(jb + 1, y1);
   }
   for (int j = jb; j < je; ++j) {
     double* ys = Ys[(j - jb) & 1];
 #pragma unroll
     for (int e = 0; e < 4; ++e) ys[tid + 256 * e] = y0[e];
     __syncthreads();
-    if (j + 2 < je) fetch(j + 2, a2);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) y0[e] = y1[e];
-    if (j + 2 < je) loady(j + 2, y1);
+    if (j + 1 < je) {
+      fetch(j + 1, a1);
+      loady(j + 1, y0);
+    }
     const double* yk = ys + k0 * MC;
 #pragma unroll
-    for (int kk = 0; kk < KW; ++kk) {
-      const double a = a0[kk];
+    for (int kk = 0; kk < KH; ++kk) {
 #pragma unroll
-      for (int c = 0; c < MC; ++c) acc[c] += a * yk[kk * MC + c];
+      for (int c = 0; c < MC; ++c) {
+        const double y = yk[kk * MC + c];
+        acc0[c] += a0[kk] * y;
+        acc1[c] += a0[KH + kk] * y;
+      }
     }
 #pragma unroll
-    for (int kk = 0; kk < KW; ++kk) {
-      a0[kk] = a1[kk];
-      a1[kk] = a2[kk];
-    }
+    for (int kk = 0; kk < 2 * KH; ++kk) a0[kk] = a1[kk];
   }
-  __syncthreads();
-  if (wv > 0) {
+  // the two halves of a wave hold the same rows at different k
 #pragma unroll
-    for (int c = 0; c < MC; ++c) Part[wv - 1][r * MC + c] = acc[c];
+  for (int c = 0; c < MC; ++c) {
+    acc0[c] += __shfl_xor(acc0[c], 32, 64);
+    acc1[c] += __shfl_xor(acc1[c], 32, 64);
   }
   __syncthreads();
-  if (wv == 0 && r < bi) {
+  if (wv > 0 && h == 0) {
 #pragma unroll
     for (int c = 0; c < MC; ++c) {
-      if (c < mc) {
-        const double sum = acc[c] + Part[0][r * MC + c] + Part[1][r * MC + c] + Part[2][r * MC + c];
-        const int64_t row = i0 + r;
-        if (nsplit == 1)
-          R[row * ldr + c0 + c] = ABS ? sum : B[row * ldb + c0 + c] - sum;
-        else
-          P[((int64_t)sp * n + row) * mp + c0 + c] = sum;
+      Part[wv - 1][p * MC + c] = acc0[c];
+      Part[wv - 1][(p + 32) * MC + c] = acc1[c];
+    }
+  }
+  __syncthreads();
+  if (wv == 0) {
+    // lane l < 32 finishes row p, lane l >= 32 row p + 32 (both halves hold both sums)
+    const int row_in = h ? p + 32 : p;
+    if (row_in < bi) {
+#pragma unroll
+      for (int c = 0; c < MC; ++c) {
+        if (c < mc) {
+          const double own = h ? acc1[c] : acc0[c];
+          const double sum = own + Part[0][row_in * MC + c] + Part[1][row_in * MC + c] + Part[2][row_in * MC + c];
+          const int64_t row = i0 + row_in;
+          if (nsplit == 1)
+            R[row * ldr + c0 + c] = ABS ? sum : B[row * ldb + c0 + c] - sum;
+          else
+            P[((int64_t)sp * n + row) * mp + c0 + c] = sum;
+        }
       }
     }
   }
