@@ -55,11 +55,21 @@ llh_status wrong_rows(llh_error* err, size_t expected, size_t actual) {
   return set_error(err, LLH_ERR_WRONG_ROWS, expected, actual, -1, buf, 0);
 }
 
+// Waits for the context stream and the factor's lookahead streams: before a
+// buffer any of them may still read is freed (a call that failed while
+// enqueueing may have left side-stream work unjoined).
+hipError_t ctx_quiesce(llh_context* ctx) {
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && ctx->la.side != nullptr) e = hipStreamSynchronize(ctx->la.side);
+  if (e == hipSuccess && ctx->la.fwd != nullptr) e = hipStreamSynchronize(ctx->la.fwd);
+  return e;
+}
+
 hipError_t ctx_buffer(llh_context* ctx, int slot, size_t bytes, void** out) {
   auto& b = ctx->scratch[slot];
   if (b.bytes < bytes) {
     if (b.ptr != nullptr) {
-      hipError_t e = hipStreamSynchronize(ctx->stream);
+      hipError_t e = ctx_quiesce(ctx);
       if (e != hipSuccess) return e;
       (void)hipFree(b.ptr);
       b.ptr = nullptr;
@@ -78,7 +88,7 @@ static hipError_t grow(llh_context* ctx, llh_context::Buffer& b, size_t bytes, b
   *fresh = false;
   if (b.bytes >= bytes) return hipSuccess;
   if (b.ptr != nullptr) {
-    hipError_t e = hipStreamSynchronize(ctx->stream);
+    hipError_t e = ctx_quiesce(ctx);
     if (e != hipSuccess) return e;
     (void)hipFree(b.ptr);
     b.ptr = nullptr;

@@ -47,6 +47,8 @@ llh_status not_square(llh_error* err, size_t rows, size_t cols);
 llh_status not_pd(llh_error* err, int64_t col);
 llh_status wrong_rows(llh_error* err, size_t expected, size_t actual);
 hipError_t ctx_buffer(llh_context* ctx, int slot, size_t bytes, void** out);
+// Synchronizes the context stream and the lookahead streams.
+hipError_t ctx_quiesce(llh_context* ctx);
 // Grow-only diagonal-inverse buffer for order n (elements of type size es).
 hipError_t ctx_dinv(llh_context* ctx, int n, size_t es, void** out);
 // Solve-sync words sized for order n and m right-hand sides; fills *sync.

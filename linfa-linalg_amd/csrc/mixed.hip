@@ -47,7 +47,7 @@ bool low_precision(llh_precision p) { return p == LLH_PREC_F32 || p == LLH_PREC_
 hipError_t grow_buffer(llh_context* ctx, bool owns, llh_context::Buffer* borrowed, void** ptr, size_t* bytes,
                        size_t want) {
   if (*bytes >= want) return hipSuccess;
-  hipError_t e = hipStreamSynchronize(ctx->stream);
+  hipError_t e = ctx_quiesce(ctx);
   if (e != hipSuccess) return e;
   if (owns) {
     if (*ptr) (void)hipFree(*ptr);
