@@ -20,7 +20,9 @@ constexpr int32_t kNoFail = 0x7fffffff;
 // k-slabs per read-modify-write of C but a longer panel chain.  Measured solves:
 // n = 32768 f64 229 / 220 / 223 / 226 ms at 512 / 1024 / 1536 / 2048; n = 16384
 // f64 41.6 / 42.5 ms at 512 / 1024; the f16-operand factor, whose SYRK is fast and
-// whose panel chain sets the pace, 87 / 95 ms at 512 / 1024 (n = 32768); the FP32
+// whose panel chain sets the pace, 87 / 95 ms at 512 / 1024 (n = 32768), and later
+// FP16+IR 88.5 / 78.1 / 70.8 ms at 256 / 384 / 512 (narrower panels pass over the
+// f32 trailing matrix more often); the FP32
 // refined solve 166 / 158 ms at 512 / 1024 (n = 32768: its rank-512 SYRKs cost
 // more than the shorter panel chain saves).
 constexpr int kNB = 64;
