@@ -24,7 +24,8 @@ constexpr int32_t kNoFail = 0x7fffffff;
 // FP16+IR 88.5 / 78.1 / 70.8 ms at 256 / 384 / 512 (narrower panels pass over the
 // f32 trailing matrix more often); the FP32
 // refined solve 166 / 158 ms at 512 / 1024 (n = 32768: its rank-512 SYRKs cost
-// more than the shorter panel chain saves).
+// more than the shorter panel chain saves), later 134.6 / 133.4 / 134.6 ms at
+// 768 / 1024 / 1536.
 constexpr int kNB = 64;
 constexpr int kNBO = 512;
 constexpr int kNBO_LARGE = 1024;
